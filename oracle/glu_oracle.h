@@ -1,0 +1,150 @@
+/*
+ * glu_oracle.h -- CPU restatement of GLU's Fourier-accelerated gauge fixing.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the parity checker for the HIP path in
+ * glu_amd/: only tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py may load it, and only to check or to time a CPU baseline.  The
+ * product library never links or calls it.
+ *
+ * Conventions (all as in GLU, see the citations next to each function):
+ *   - lattice sites are lexicographic with x fastest, t slowest
+ *     (Geometry/geometry.c:28-36);
+ *   - a link is 9 complex doubles, row-major, O[a*3+b] = U_ab, stored as 18
+ *     interleaved re/im doubles (Headers/GLU_definitions.h:102);
+ *   - a gauge field is double U[V][ND][18] (site-major, then direction);
+ *   - the packed FFT "4-field" is 4 complex arrays of V entries each,
+ *     double F[4][V][2] (Gfix/FACG.c:57-60, TRUE_HERM = 4).
+ *
+ * Reductions follow GLU's OpenMP structure: the site loop is cut into
+ * "virtual threads" exactly like libgomp's static schedule, each virtual
+ * thread accumulates its own partial in site order (plain or Kahan, as GLU
+ * does), and partials are added in thread order.  With nth equal to GLU's
+ * OMP_NUM_THREADS the summation order is GLU's.
+ *
+ * Parity status of each part is recorded in DESIGN.md and in the header of
+ * tests/test_oracle_pins.py.
+ */
+#ifndef GLU_ORACLE_H
+#define GLU_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_ND 4
+#define OR_NC 3
+#define OR_LINK 18 /* doubles per link */
+
+/* GLU's failure sentinel for FACG/FASD (Gfix/FACG.c:381) */
+#define OR_GF_FAIL 123456789u
+
+typedef struct {
+  size_t dims[OR_ND];
+  size_t V, Lcu;
+  size_t *fwd; /* V*ND : fwd[i*ND+mu] = site i + mu-hat */
+  size_t *bwd; /* V*ND : bwd[i*ND+mu] = site i - mu-hat */
+  int nth;     /* virtual threads, GLU's Latt.Nthreads */
+} or_lat;
+
+/* per-iteration trajectory record (optional, any pointer may be NULL) */
+typedef struct {
+  size_t cap;      /* capacity of each array */
+  size_t n;        /* entries written */
+  double *theta;   /* theta_k measured before update k (GLU's *tr)  */
+  double *alpha;   /* step applied at update k (line-search or fixed) */
+  double *beta;    /* CG beta used at update k (0 for the SD start row) */
+  double *func;    /* gauge functional after update k (costly)       */
+} or_trace;
+
+/* ---------------- geometry (Geometry/geometry.c, init.c) ---------------- */
+int    or_lat_init(or_lat *L, const size_t dims[OR_ND], int nth);
+void   or_lat_free(or_lat *L);
+double or_psq(const or_lat *L, size_t i, int D); /* gen_p_sq geometry.c:170 */
+
+/* ---------------- SU(3) kernels (Matrix_Ops, Gfix/CG.c, gtrans.c) ------- */
+void   or_gram_reunit(double *U18);
+void   or_set_gauge_matrix(double *g18, const double *F, size_t V,
+                           double alpha, size_t i);
+void   or_gtransform_local(const double *a, double *b, const double *c);
+double or_retrace_abc_dag(const double *a, const double *b, const double *c);
+void   or_herm_proj_short(double *Q10, const double *U18);
+double or_approx_minimum(const double alphas[3], const double f[3]);
+
+/* ---------------- measurements (Field/plaqs_links.c, gftests.c) --------- */
+double or_plaquette(const or_lat *L, const double *U);
+double or_links(const or_lat *L, const double *U, double *maxtr);
+void   or_all_links(const or_lat *L, const double *U, double *sp, double *t);
+double or_functional(const or_lat *L, const double *U);
+double or_theta_lin(const or_lat *L, const double *U, int max_dir);
+void   or_latt_reunit(const or_lat *L, double *U);
+
+/* ---------------- derivative (Gfix/lin_derivs.c, FACG.c) ---------------- */
+double or_fast_deriv(const or_lat *L, const double *U, size_t i, double *sum10);
+double or_latt_deriv(const or_lat *L, const double *U, size_t i, int max_dir,
+                     double *sum10);
+double or_fa_deriv(const or_lat *L, const double *U, double *F); /* theta */
+
+/* ---------------- FFT ---------------------------------------------------- */
+/* unnormalised complex DFT over the first nd dims (x fastest) of `dims`,
+   sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD, applied to `nb` arrays of
+   prod(dims[0..nd-1]) complex entries laid out back to back */
+void   or_fft(double *data, const size_t *dims, int nd, int sign, size_t nb);
+void   or_dft_naive(double *data, const size_t *dims, int nd, int sign);
+void   or_psq_table(const or_lat *L, int D, double *psq); /* MAX/p^2 */
+void   or_fourier_accelerate(const or_lat *L, double *F, const double *psq,
+                             int D, size_t nb);
+
+/* ---------------- Landau (Gfix/FACG.c, line_search.c, Landau.c) --------- */
+void   or_egauge_landau(const or_lat *L, double *U, const double *F,
+                        double alpha);
+double or_line_search_landau(const or_lat *L, double *U, const double *F,
+                             double *red);
+size_t or_facg(const or_lat *L, double *U, double gf_alpha, double acc,
+               size_t max_iters, double *theta, or_trace *tr);
+size_t or_fasd(const or_lat *L, double *U, double gf_alpha, double acc,
+               size_t max_iters, double *theta, or_trace *tr);
+
+/* ---------------- Coulomb (Gfix/CFACG.c, Coulomb.c) --------------------- */
+size_t or_coulomb(const or_lat *L, double *U, double gf_alpha, double acc,
+                  size_t max_iters, int facg, uint32_t seed,
+                  size_t *slice_iters, double *slice_theta);
+
+/* ---------------- RNG + random gauge transform (Utils/par_*.c) ---------- */
+typedef struct or_rng or_rng;
+or_rng *or_rng_new(uint32_t seed, int nth);
+void    or_rng_free(or_rng *R);
+double  or_rng_dbl(or_rng *R, int thread);
+void    or_sunitary_gen(or_rng *R, int thread, double *Z18);
+void    or_random_gtrans(const or_lat *L, or_rng *R, double *U);
+void    or_gtransform(const or_lat *L, double *U, const double *G);
+
+/* ---------------- NERSC I/O (IO/read_headers.c, readers.c) -------------- */
+/* reads a NERSC file; returns 0 on success.  U is allocated by the caller
+   with V*ND*18 doubles once dims are known: call with U == NULL first to
+   get dims/header values, then again with storage. */
+typedef struct {
+  size_t dims[OR_ND];
+  double plaquette, link_trace;
+  uint32_t checksum_header, checksum_computed;
+  int two_rows;     /* 4D_SU3_GAUGE (3x2) vs 4D_SU3_GAUGE_3x3 */
+  int double_prec;  /* IEEE64 vs IEEE32 */
+  int big_endian;
+  long data_offset;
+} or_nersc_info;
+int or_read_nersc(const char *path, or_nersc_info *info, double *U);
+int or_write_nersc(const char *path, const or_lat *L, const double *U,
+                   int two_rows);
+
+/* ---------------- input generation (not on the hot path) ---------------- */
+/* quenched Wilson heatbath + overrelaxation, Cabibbo-Marinari with
+   Kennedy-Pendleton SU(2) heatbath; own seeded generator */
+void or_heatbath(const or_lat *L, double *U, double beta, int sweeps,
+                 int nor, uint64_t seed, int cold_start);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

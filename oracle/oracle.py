@@ -1,0 +1,315 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (glu_amd) never
+imports it.
+
+Arrays use GLU's layouts: links ``U[V, 4, 18]`` float64 (site-major, x
+fastest, row-major 3x3 complex as re/im pairs) and packed fields
+``F[4, V]`` complex128.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+SZ = C.c_size_t
+DP = C.POINTER(C.c_double)
+
+
+class Lat(C.Structure):
+    _fields_ = [("dims", SZ * 4), ("V", SZ), ("Lcu", SZ),
+                ("fwd", C.POINTER(SZ)), ("bwd", C.POINTER(SZ)), ("nth", C.c_int)]
+
+
+class Trace(C.Structure):
+    _fields_ = [("cap", SZ), ("n", SZ), ("theta", DP), ("alpha", DP),
+                ("beta", DP), ("func", DP)]
+
+
+class NerscInfo(C.Structure):
+    _fields_ = [("dims", SZ * 4), ("plaquette", C.c_double),
+                ("link_trace", C.c_double), ("checksum_header", C.c_uint32),
+                ("checksum_computed", C.c_uint32), ("two_rows", C.c_int),
+                ("double_prec", C.c_int), ("big_endian", C.c_int),
+                ("data_offset", C.c_long)]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        LP = C.POINTER(Lat)
+        sig = {
+            "or_lat_init": (C.c_int, [LP, C.POINTER(SZ), C.c_int]),
+            "or_lat_free": (None, [LP]),
+            "or_psq": (C.c_double, [LP, SZ, C.c_int]),
+            "or_gram_reunit": (None, [DP]),
+            "or_set_gauge_matrix": (None, [DP, DP, SZ, C.c_double, SZ]),
+            "or_gtransform_local": (None, [DP, DP, DP]),
+            "or_retrace_abc_dag": (C.c_double, [DP, DP, DP]),
+            "or_herm_proj_short": (None, [DP, DP]),
+            "or_approx_minimum": (C.c_double, [DP, DP]),
+            "or_plaquette": (C.c_double, [LP, DP]),
+            "or_links": (C.c_double, [LP, DP, DP]),
+            "or_all_links": (None, [LP, DP, DP, DP]),
+            "or_functional": (C.c_double, [LP, DP]),
+            "or_theta_lin": (C.c_double, [LP, DP, C.c_int]),
+            "or_latt_reunit": (None, [LP, DP]),
+            "or_fast_deriv": (C.c_double, [LP, DP, SZ, DP]),
+            "or_latt_deriv": (C.c_double, [LP, DP, SZ, C.c_int, DP]),
+            "or_fa_deriv": (C.c_double, [LP, DP, DP]),
+            "or_fft": (None, [DP, C.POINTER(SZ), C.c_int, C.c_int, SZ]),
+            "or_dft_naive": (None, [DP, C.POINTER(SZ), C.c_int, C.c_int]),
+            "or_psq_table": (None, [LP, C.c_int, DP]),
+            "or_fourier_accelerate": (None, [LP, DP, DP, C.c_int, SZ]),
+            "or_egauge_landau": (None, [LP, DP, DP, C.c_double]),
+            "or_line_search_landau": (C.c_double, [LP, DP, DP, DP]),
+            "or_facg": (SZ, [LP, DP, C.c_double, C.c_double, SZ, DP,
+                             C.POINTER(Trace)]),
+            "or_fasd": (SZ, [LP, DP, C.c_double, C.c_double, SZ, DP,
+                             C.POINTER(Trace)]),
+            "or_coulomb": (SZ, [LP, DP, C.c_double, C.c_double, SZ, C.c_int,
+                                C.c_uint32, C.POINTER(SZ), DP]),
+            "or_rng_new": (C.c_void_p, [C.c_uint32, C.c_int]),
+            "or_rng_free": (None, [C.c_void_p]),
+            "or_rng_dbl": (C.c_double, [C.c_void_p, C.c_int]),
+            "or_sunitary_gen": (None, [C.c_void_p, C.c_int, DP]),
+            "or_random_gtrans": (None, [LP, C.c_void_p, DP]),
+            "or_gtransform": (None, [LP, DP, DP]),
+            "or_read_nersc": (C.c_int, [C.c_char_p, C.POINTER(NerscInfo), DP]),
+            "or_write_nersc": (C.c_int, [C.c_char_p, LP, DP, C.c_int]),
+            "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
+                                   C.c_uint64, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"] and a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(DP)
+
+
+class Lattice:
+    """Geometry + neighbour tables for a 4D periodic lattice (x fastest)."""
+
+    def __init__(self, dims, nth: int | None = None):
+        self.dims = tuple(int(d) for d in dims)
+        self._lat = Lat()
+        d = (SZ * 4)(*self.dims)
+        if nth is None:
+            nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        if lib().or_lat_init(C.byref(self._lat), d, int(nth)) != 0:
+            raise MemoryError("or_lat_init failed")
+        self.V = int(self._lat.V)
+        self.Lcu = int(self._lat.Lcu)
+        self.nth = int(nth)
+
+    def __del__(self):
+        try:
+            lib().or_lat_free(C.byref(self._lat))
+        except Exception:
+            pass
+
+    @property
+    def ref(self):
+        return C.byref(self._lat)
+
+    def fwd(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._lat.fwd, (self.V, 4)).copy()
+
+    def bwd(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._lat.bwd, (self.V, 4)).copy()
+
+    # measurements
+    def plaquette(self, U):
+        return lib().or_plaquette(self.ref, _dp(U))
+
+    def links(self, U):
+        mx = C.c_double()
+        tr = lib().or_links(self.ref, _dp(U), C.byref(mx))
+        return tr, mx.value
+
+    def functional(self, U):
+        return lib().or_functional(self.ref, _dp(U))
+
+    def theta_lin(self, U, max_dir=4):
+        return lib().or_theta_lin(self.ref, _dp(U), max_dir)
+
+    def psq(self, D=4):
+        n = self.V if D == 4 else self.Lcu
+        out = np.empty(n)
+        lib().or_psq_table(self.ref, D, _dp(out))
+        return out
+
+    def fa_deriv(self, U):
+        F = np.empty((4, self.V), np.complex128)
+        th = lib().or_fa_deriv(self.ref, _dp(U), _dp(F))
+        return th, F
+
+    def fourier_accelerate(self, F, D=4):
+        psq = self.psq(D)
+        F = np.ascontiguousarray(F, np.complex128).copy()
+        lib().or_fourier_accelerate(self.ref, _dp(F), _dp(psq), D, 4)
+        return F
+
+    def egauge(self, U, F, alpha):
+        lib().or_egauge_landau(self.ref, _dp(U), _dp(F), alpha)
+
+    def reunit(self, U):
+        lib().or_latt_reunit(self.ref, _dp(U))
+
+    def random_gtrans(self, U, seed: int, nth: int | None = None):
+        r = lib().or_rng_new(seed, nth or self.nth)
+        lib().or_random_gtrans(self.ref, r, _dp(U))
+        lib().or_rng_free(r)
+
+    def gtransform(self, U, G):
+        lib().or_gtransform(self.ref, _dp(U), _dp(G))
+
+    def heatbath(self, beta, sweeps, nor=4, seed=1, U=None):
+        cold = U is None
+        if U is None:
+            U = np.zeros((self.V, 4, 18))
+        lib().or_heatbath(self.ref, _dp(U), beta, sweeps, nor, seed, int(cold))
+        return U
+
+    def facg(self, U, gf_alpha=0.08, acc=1e-14, max_iters=1000, trace=0,
+             functional=False, sd=False):
+        th = C.c_double()
+        tr = None
+        bufs = {}
+        if trace:
+            tr = Trace()
+            tr.cap = trace
+            tr.n = 0
+            for k in ("theta", "alpha", "beta") + (("func",) if functional else ()):
+                bufs[k] = np.zeros(trace)
+                setattr(tr, k, _dp(bufs[k]))
+        f = lib().or_fasd if sd else lib().or_facg
+        it = f(self.ref, _dp(U), gf_alpha, acc, max_iters, C.byref(th),
+               C.byref(tr) if tr is not None else None)
+        out = {"iters": int(it), "theta": th.value}
+        if tr is not None:
+            for k, v in bufs.items():
+                out["trace_" + k] = v[: tr.n].copy()
+        return out
+
+    def coulomb(self, U, gf_alpha=0.09, acc=1e-14, max_iters=650, facg=True,
+                seed=1):
+        T = self.dims[3]
+        its = (SZ * T)()
+        th = np.zeros(T)
+        tot = lib().or_coulomb(self.ref, _dp(U), gf_alpha, acc, max_iters,
+                               int(facg), seed, its, _dp(th))
+        return {"iters": int(tot), "slice_iters": list(its), "slice_theta": th}
+
+
+def fft(data: np.ndarray, dims, nd: int, sign: int) -> np.ndarray:
+    """Unnormalised DFT over the first `nd` axes of `dims` (x fastest) of
+    every row of `data` (shape [nb, prod(dims[:nd])])."""
+    a = np.ascontiguousarray(data, np.complex128).copy()
+    nb = a.shape[0] if a.ndim > 1 else 1
+    d = (SZ * 4)(*(list(dims) + [1] * (4 - len(dims))))
+    lib().or_fft(_dp(a), d, nd, sign, nb)
+    return a
+
+
+def dft_naive(data: np.ndarray, dims, nd: int, sign: int) -> np.ndarray:
+    a = np.ascontiguousarray(data, np.complex128).copy()
+    d = (SZ * 4)(*(list(dims) + [1] * (4 - len(dims))))
+    lib().or_dft_naive(_dp(a), d, nd, sign)
+    return a
+
+
+def read_nersc(path: str):
+    info = NerscInfo()
+    rc = lib().or_read_nersc(path.encode(), C.byref(info), None)
+    if rc != 0:
+        raise IOError(f"NERSC header read failed ({rc}): {path}")
+    V = int(np.prod(list(info.dims)))
+    U = np.zeros((V, 4, 18))
+    rc = lib().or_read_nersc(path.encode(), C.byref(info), _dp(U))
+    if rc != 0:
+        raise IOError(f"NERSC data read failed ({rc}): {path}")
+    return U, info
+
+
+def write_nersc(path: str, lat: Lattice, U: np.ndarray, two_rows=False):
+    rc = lib().or_write_nersc(path.encode(), lat.ref, _dp(U), int(two_rows))
+    if rc != 0:
+        raise IOError(f"NERSC write failed ({rc})")
+
+
+def rng_draws(seed: int, nth: int, thread: int, n: int) -> np.ndarray:
+    r = lib().or_rng_new(seed, nth)
+    out = np.array([lib().or_rng_dbl(r, thread) for _ in range(n)])
+    lib().or_rng_free(r)
+    return out
+
+
+# single-matrix helpers (links are 18-double vectors)
+def gram_reunit(U18):
+    u = np.ascontiguousarray(U18, np.float64).copy()
+    lib().or_gram_reunit(_dp(u))
+    return u
+
+
+def gtransform_local(a, b, c):
+    b = np.ascontiguousarray(b, np.float64).copy()
+    lib().or_gtransform_local(_dp(np.ascontiguousarray(a, np.float64)), _dp(b),
+                              _dp(np.ascontiguousarray(c, np.float64)))
+    return b
+
+
+def retrace_abc_dag(a, b, c):
+    return lib().or_retrace_abc_dag(*[_dp(np.ascontiguousarray(x, np.float64))
+                                      for x in (a, b, c)])
+
+
+def herm_proj_short(U18):
+    q = np.zeros(10)
+    lib().or_herm_proj_short(_dp(q), _dp(np.ascontiguousarray(U18, np.float64)))
+    return q
+
+
+def set_gauge_matrix(F, alpha, i):
+    F = np.ascontiguousarray(F, np.complex128)
+    g = np.zeros(18)
+    lib().or_set_gauge_matrix(_dp(g), _dp(F), F.shape[1], alpha, i)
+    return g
+
+
+def approx_minimum(alphas, f):
+    a = np.ascontiguousarray(alphas, np.float64)
+    b = np.ascontiguousarray(f, np.float64)
+    return lib().or_approx_minimum(_dp(a), _dp(b))
+
+
+def sunitary(seed: int, nth: int, thread: int, n: int):
+    r = lib().or_rng_new(seed, nth)
+    out = np.zeros((n, 18))
+    for k in range(n):
+        lib().or_sunitary_gen(r, thread, _dp(out[k]))
+    lib().or_rng_free(r)
+    return out

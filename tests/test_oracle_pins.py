@@ -1,0 +1,363 @@
+"""Pins of the CPU oracle (oracle/liboracle.so) against known answers,
+closed forms and invariants.
+
+The reference's own build needs generated code (config.h) and FFTW, which
+this image lacks, so the reference cannot be run here; each oracle function
+is pinned instead by a check that a plausible mistake (dropped term, wrong
+sign or index, transposed operand) would fail:
+
+  NERSC reader, plaquette, link trace  CONF/lat..400 header known answers
+                                       (plaquette, link trace, checksum)
+  p^2 table                            gen_p_sq(0) == 1 (tests/geom_test.c:
+                                       126-133) and the closed form
+  FFT                                  definition-level DFT, numpy.fft,
+                                       delta/plane-wave closed forms
+  derivative                           fast == generic Hermitian projection
+                                       (lin_derivs.c:56-70 vs :75-193),
+                                       theta_lin == 2 theta_FA, unit gauge,
+                                       sum_x Delta(x) == 0, abelian closed form
+  reunitarisation, set_gauge_matrix    unitarity, det 1, idempotence, alpha 0
+  gtransform_local, Re tr(abc^dag)     numpy matrix products on SU(3)
+                                       (tests/U_Nops_test.c:378-389 pattern)
+  random gauge transform               plaquette/Polyakov-loop invariance
+                                       (tests/units.c:210-239 pattern)
+  line search probes                   equal the functional after a trial
+                                       egauge at each probe step
+  approx_minimum                       exact parabola vertex and the clamps
+  FACG / FASD / Coulomb loops          pure-gauge input -> unit links,
+                                       theta < acc, plaquette preserved,
+                                       functional decreasing
+
+Parity unpinned: the MWC_4096 stream values (only its statistics are
+checked) and the exact FACG trajectory (theta_k, alpha_k, beta_k), which
+only a run of the reference could pin.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from conftest import GOLDEN
+
+CONF = os.path.join(GOLDEN, "conf_lat400.nersc")
+
+
+def as_mats(U):
+    """[..., 18] doubles -> [..., 3, 3] complex"""
+    return (U[..., 0::2] + 1j * U[..., 1::2]).reshape(U.shape[:-1] + (3, 3))
+
+
+def from_mats(M):
+    out = np.empty(M.shape[:-2] + (18,))
+    f = M.reshape(M.shape[:-2] + (9,))
+    out[..., 0::2] = f.real
+    out[..., 1::2] = f.imag
+    return out
+
+
+def haar(rng, n):
+    z = (rng.standard_normal((n, 3, 3)) + 1j * rng.standard_normal((n, 3, 3)))
+    q, r = np.linalg.qr(z)
+    q = q * (np.diagonal(r, axis1=1, axis2=2) / np.abs(np.diagonal(r, axis1=1, axis2=2)))[:, None, :]
+    d = np.linalg.det(q)
+    return q / (d ** (1 / 3))[:, None, None]
+
+
+@pytest.fixture(scope="module")
+def conf():
+    U, info = O.read_nersc(CONF)
+    return U, info, O.Lattice(info.dims, nth=4)
+
+
+# ------------------------------------------------------------------ I/O ---
+def test_nersc_known_answers(conf):
+    U, info, L = conf
+    assert list(info.dims) == [4, 4, 4, 8]
+    assert info.checksum_computed == info.checksum_header == 0xF2EE7C36
+    assert abs(L.plaquette(U) - 0.5985455591) < 1e-10
+    tr, _ = L.links(U)
+    assert abs(tr - (-0.0007741846376)) < 1e-13
+
+
+def test_nersc_roundtrip(tmp_path, conf):
+    U, info, L = conf
+    p = str(tmp_path / "w.nersc")
+    O.write_nersc(p, L, U, two_rows=False)
+    U2, info2 = O.read_nersc(p)
+    assert np.array_equal(U, U2)
+    assert info2.checksum_header == info2.checksum_computed
+
+
+# ------------------------------------------------------------- geometry ---
+def test_neighbours_and_psq():
+    L = O.Lattice((4, 6, 2, 8), nth=1)
+    f, b = L.fwd(), L.bwd()
+    V = L.V
+    assert np.array_equal(b[f[:, 1], 1], np.arange(V))
+    x = np.arange(V)
+    assert np.array_equal(f[:, 0] // 4, x // 4)  # x wraps within its row
+    psq = L.psq(4)
+    assert abs(psq[0] - 16.0 / V) < 1e-15  # p^2(0) := 1
+    n = np.stack(np.unravel_index(np.arange(V), (8, 2, 6, 4))[::-1], 1)
+    p2 = 2 * (4 - np.cos(2 * np.pi * n / np.array([4, 6, 2, 8])).sum(1))
+    p2[0] = 1.0
+    assert np.allclose(psq, (16.0 / V) / p2, rtol=1e-14, atol=0)
+
+
+# ------------------------------------------------------------------ FFT ---
+@pytest.mark.parametrize("dims", [(2, 3, 4, 6), (8, 8, 2, 4), (3, 3, 3, 3), (12, 2, 2, 2)])
+def test_fft_vs_definition(dims):
+    rng = np.random.default_rng(3)
+    V = int(np.prod(dims))
+    a = rng.standard_normal(V) + 1j * rng.standard_normal(V)
+    for sign in (-1, 1):
+        f = O.fft(a[None], dims, 4, sign)[0]
+        n = O.dft_naive(a, dims, 4, sign)
+        ref = (np.fft.fftn if sign < 0 else lambda x, axes: np.fft.ifftn(x, axes=axes) * V)(
+            a.reshape(dims[::-1]), axes=(0, 1, 2, 3)).reshape(V)
+        assert np.abs(f - n).max() < 1e-12 * np.abs(n).max()
+        assert np.abs(f - ref).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_fft_closed_forms():
+    dims = (4, 6, 8, 3)
+    V = int(np.prod(dims))
+    d = np.zeros(V, complex)
+    d[0] = 1.0
+    assert np.allclose(O.fft(d[None], dims, 4, -1)[0], 1.0, atol=1e-15)
+    k = (1, 2, 3, 1)
+    n = np.stack(np.unravel_index(np.arange(V), dims[::-1])[::-1], 1)
+    pw = np.exp(2j * np.pi * (n * np.array(k) / np.array(dims)).sum(1))
+    f = O.fft(pw[None], dims, 4, -1)[0]
+    ki = k[0] + dims[0] * (k[1] + dims[1] * (k[2] + dims[2] * k[3]))
+    expect = np.zeros(V, complex)
+    expect[ki] = V
+    assert np.abs(f - expect).max() < 1e-11
+    # 3D transform of each time slice independently
+    a = np.random.default_rng(1).standard_normal((2, V)) + 0j
+    f3 = O.fft(a.reshape(2 * dims[3], -1), dims, 3, -1)
+    ref = np.fft.fftn(a.reshape(2 * dims[3], dims[2], dims[1], dims[0]), axes=(1, 2, 3))
+    assert np.abs(f3 - ref.reshape(2 * dims[3], -1)).max() < 1e-12
+
+
+# ----------------------------------------------------------- SU(3) ops ---
+def test_reunit_and_gauge_matrix():
+    rng = np.random.default_rng(5)
+    G = haar(rng, 20)
+    for g in G:
+        u = from_mats(g[None])[0]
+        r = O.gram_reunit(u)
+        assert np.abs(r - u).max() < 1e-14  # idempotent on SU(3)
+        m = as_mats(O.gram_reunit(u + 0.1 * rng.standard_normal(18)))
+        assert np.abs(m @ m.conj().T - np.eye(3)).max() < 1e-14
+        assert abs(np.linalg.det(m) - 1) < 1e-14
+    F = rng.standard_normal((4, 5)) + 1j * rng.standard_normal((4, 5))
+    assert np.abs(as_mats(O.set_gauge_matrix(F, 0.0, 2)) - np.eye(3)).max() < 1e-15
+    # small alpha: g ~ 1 + alpha X with X antihermitian traceless
+    a = 1e-6
+    X = (as_mats(O.set_gauge_matrix(F, a, 3)) - np.eye(3)) / a
+    assert np.abs(X + X.conj().T).max() < 1e-5
+    assert abs(np.trace(X)) < 1e-5
+    # the packing of set_gauge_matrix (Gfix/CG.c:73-78)
+    f = F[:, 3]
+    assert abs(X[0, 0] - 1j * f[0].imag) < 1e-5 and abs(X[1, 1] + 1j * f[0].real) < 1e-5
+    assert abs(X[0, 1] - f[1]) < 1e-5 and abs(X[2, 0] + np.conj(f[2])) < 1e-5
+    assert abs(X[2, 1] + np.conj(f[3])) < 1e-5
+
+
+def test_gtransform_and_trace_abc():
+    rng = np.random.default_rng(7)
+    A, B, Cm = haar(rng, 10), haar(rng, 10), haar(rng, 10)
+    for a, b, c in zip(A, B, Cm):
+        got = as_mats(O.gtransform_local(from_mats(a[None])[0], from_mats(b[None])[0],
+                                         from_mats(c[None])[0]))
+        assert np.abs(got - a @ b @ c.conj().T).max() < 1e-14
+        t = O.retrace_abc_dag(from_mats(a[None])[0], from_mats(b[None])[0], from_mats(c[None])[0])
+        assert abs(t - np.trace(a @ b @ c.conj().T).real) < 1e-14
+
+
+def test_herm_proj():
+    rng = np.random.default_rng(9)
+    for u in haar(rng, 10):
+        q = O.herm_proj_short(from_mats(u[None])[0])
+        Q = (u - u.conj().T) / 2j
+        Q = Q - np.trace(Q) / 3 * np.eye(3)
+        assert np.allclose(q[[0, 2, 3, 4, 5, 6, 8, 9]],
+                           [Q[0, 0].real, Q[0, 1].real, Q[0, 1].imag, Q[0, 2].real,
+                            Q[0, 2].imag, Q[1, 1].real, Q[1, 2].real, Q[1, 2].imag], atol=1e-15)
+
+
+def test_approx_minimum():
+    al = np.array([0.0, 0.15, 0.30])
+    for x0 in (0.05, 0.12, 0.29):
+        f = 3.0 * (al - x0) ** 2 - 1.0
+        assert abs(O.approx_minimum(al, f) - x0) < 1e-13
+    # no curvature test in GLU: a concave fit is clamped like a convex one
+    assert O.approx_minimum(al, -(al - 0.5) ** 2) == 0.30
+    assert O.approx_minimum(al, -(al + 0.5) ** 2) == 0.08
+    assert O.approx_minimum(al, (al - 0.9) ** 2) == 0.30  # beyond the last probe
+    assert O.approx_minimum(al, (al + 0.2) ** 2) == 0.08  # below zero
+    assert O.approx_minimum(al, np.zeros(3)) == 0.08  # 0/0
+
+
+# ------------------------------------------------------------ derivative ---
+def test_derivative_pins(conf):
+    U, info, L = conf
+    th_fa, F = L.fa_deriv(U)
+    assert abs(L.theta_lin(U) - 2 * th_fa) < 1e-13 * th_fa
+    # divergence of a periodic field sums to zero
+    assert np.abs(F.sum(axis=1)).max() < 1e-11
+    # generic vs specialised derivative, site by site, and the packing
+    lib = O.lib()
+    import ctypes as C
+    for i in (0, 17, 511):
+        s1, s2 = np.zeros(10), np.zeros(10)
+        a = lib.or_fast_deriv(L.ref, O._dp(U), i, O._dp(s1))
+        b = lib.or_latt_deriv(L.ref, O._dp(U), i, 4, O._dp(s2))
+        assert abs(a - b) < 1e-13 * max(1, b)
+        assert np.allclose(s1, s2, atol=1e-14)
+        assert abs(F[0, i] - (1j * s1[0] - s1[6])) < 1e-14
+        assert abs(F[1, i] - (1j * s1[2] - s1[3])) < 1e-14
+        assert abs(F[3, i] - (1j * s1[8] - s1[9])) < 1e-14
+    # unit gauge: zero derivative
+    unit = np.zeros_like(U)
+    unit[..., [0, 8, 16]] = 1.0
+    th0, F0 = L.fa_deriv(unit)
+    assert th0 == 0.0 and np.abs(F0).max() == 0.0
+
+
+def test_derivative_abelian_closed_form():
+    """U_mu(x) = diag(e^{i a}, e^{i b}, e^{-i(a+b)}) with phases depending on
+    x; Delta(x) = sum_mu [A_mu(x-mu) - A_mu(x)], A = traceless diag(sin)."""
+    L = O.Lattice((4, 4, 4, 4), nth=2)
+    rng = np.random.default_rng(11)
+    ph = rng.uniform(-1, 1, (L.V, 4, 2))
+    d = np.stack([ph[..., 0], ph[..., 1], -ph[..., 0] - ph[..., 1]], -1)
+    U = np.zeros((L.V, 4, 18))
+    for k in range(3):
+        U[..., 8 * k] = np.cos(d[..., k])
+        U[..., 8 * k + 1] = np.sin(d[..., k])
+    A = np.sin(d)
+    A = A - A.mean(-1, keepdims=True)
+    b = L.bwd()
+    Delta = sum(A[b[:, mu], mu] - A[:, mu] for mu in range(4))
+    th, F = L.fa_deriv(U)
+    assert np.allclose(F[0], 1j * Delta[:, 0] - Delta[:, 1], atol=1e-14)
+    assert np.abs(F[1:]).max() < 1e-15
+    # GLU's theta counts 4 tr(sum^2) of the short form = 2 tr(Delta^2)
+    assert abs(th - 2 * (Delta ** 2).sum() / (3 * L.V)) < 1e-13
+
+
+# ------------------------------------------------------------- transforms ---
+def polyakov(L, U, mu):
+    M = as_mats(U)
+    dims = L.dims
+    f = L.fwd()
+    tot = 0
+    for x in range(L.V):
+        P = np.eye(3, dtype=complex)
+        y = x
+        for _ in range(dims[mu]):
+            P = P @ M[y, mu]
+            y = f[y, mu]
+        tot += np.trace(P)
+    return tot / L.V
+
+
+def test_random_gtrans_invariance(conf):
+    U, info, L = conf
+    G = U.copy()
+    L.random_gtrans(G, seed=3, nth=4)
+    assert abs(L.plaquette(G) - L.plaquette(U)) < 1e-14
+    for mu in (0, 3):
+        assert abs(polyakov(L, G, mu) - polyakov(L, U, mu)) < 1e-13
+    assert np.abs(G - U).max() > 0.1
+    m = as_mats(G.reshape(-1, 18))
+    assert np.abs(m @ np.conj(np.swapaxes(m, 1, 2)) - np.eye(3)).max() < 1e-14
+
+
+def test_rng_statistics():
+    x = O.rng_draws(5, 3, 1, 20000)
+    assert 0.49 < x.mean() < 0.51 and x.min() >= 0 and x.max() < 1
+    assert not np.array_equal(O.rng_draws(5, 3, 0, 5), O.rng_draws(5, 3, 1, 5))
+    Z = as_mats(O.sunitary(1, 1, 0, 500))
+    assert np.abs(np.linalg.det(Z) - 1).max() < 1e-13
+    assert abs(np.trace(Z, axis1=1, axis2=2).mean()) < 0.15  # Haar-like: E tr = 0
+
+
+# ------------------------------------------------------------ line search ---
+def test_line_search_probes_equal_trial_functionals(conf):
+    U, info, L = conf
+    th, F = L.fa_deriv(U)
+    F = L.fourier_accelerate(F)
+    vals = []
+    for a in (0.0, 0.15, 0.30):
+        W = U.copy()
+        L.egauge(W, F, a)
+        vals.append(-4 * 3 * L.V * (1 - L.functional(W)))
+    red = np.zeros(3 + 8 * L.nth)
+    W = U.copy()
+    step = O.lib().or_line_search_landau(L.ref, O._dp(W), O._dp(F), O._dp(red))
+    got = np.array([-sum(red[j + 8 * t] for t in range(L.nth)) for j in range(3)])
+    assert np.allclose(got, vals, rtol=1e-14, atol=1e-10)
+    assert abs(step - O.approx_minimum([0, 0.15, 0.3], got)) == 0.0
+    W2 = U.copy()
+    L.egauge(W2, F, step)
+    assert np.array_equal(W, W2)
+
+
+# ------------------------------------------------------------- full loops ---
+def pure_gauge(L, seed):
+    U = np.zeros((L.V, 4, 18))
+    U[..., [0, 8, 16]] = 1.0
+    L.random_gtrans(U, seed=seed)
+    return U
+
+
+def test_facg_pure_gauge_converges_to_unit():
+    L = O.Lattice((4, 4, 4, 8), nth=4)
+    U = pure_gauge(L, 21)
+    r = L.facg(U, 0.08, 1e-14, 1000)
+    assert r["theta"] < 1e-14 and 0 < r["iters"] < 1000
+    assert abs(L.functional(U)) < 1e-12
+    assert abs(L.plaquette(U) - 1.0) < 1e-13
+
+
+def test_facg_heatbath_properties():
+    L = O.Lattice((6, 6, 6, 6), nth=4)
+    U = L.heatbath(6.0, 30, 2, seed=2)
+    p0 = L.plaquette(U)
+    L.random_gtrans(U, seed=9)
+    r = L.facg(U, 0.08, 1e-14, 1000, trace=2000, functional=True)
+    assert r["theta"] < 1e-14
+    assert abs(L.plaquette(U) - p0) < 2e-14
+    assert L.theta_lin(U) < 2e-14
+    f = r["trace_func"]
+    assert np.all(np.diff(f) < 1e-12)  # the line search never climbs
+
+
+def test_fasd_converges():
+    L = O.Lattice((4, 4, 4, 4), nth=2)
+    U = L.heatbath(6.0, 20, 2, seed=4)
+    L.random_gtrans(U, seed=5)
+    r = L.facg(U, 0.08, 1e-12, 3000, sd=True)
+    assert r["theta"] < 1e-12 and r["iters"] < 3000
+
+
+def test_coulomb_properties():
+    L = O.Lattice((4, 4, 4, 6), nth=2)
+    U = L.heatbath(6.0, 20, 2, seed=6)
+    p0 = L.plaquette(U)
+    G = U.copy()
+    L.random_gtrans(G, seed=7)
+    r = L.coulomb(G, 0.09, 1e-14, 650)
+    assert max(r["slice_theta"]) < 1e-14
+    assert abs(L.plaquette(G) - p0) < 1e-13
+    # Coulomb condition: the spatial lattice divergence vanishes
+    assert L.theta_lin(G, max_dir=3) < 1e-13
+    # pure gauge: every spatial link becomes the unit matrix
+    P = pure_gauge(L, 8)
+    L.coulomb(P, 0.09, 1e-14, 650)
+    assert np.abs(P[:, :3, [0, 8, 16]] - 1).max() < 1e-12
