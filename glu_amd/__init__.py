@@ -1,0 +1,230 @@
+"""glu_amd -- MI355X (gfx950) gauge fixing behind GLU's C entry points.
+
+The product is the C-ABI shared library ``libglu_gfix.so`` (declared in
+include/glu_gfix.h).  This module is a thin ctypes binding used by tests and
+benchmarks; it loads the in-tree library and fails loudly when it is
+missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libglu_gfix.so")
+
+LANDAU, COULOMB = 0, 1
+FP64, MIXED = 0, 1
+CG, SD = 0, 1
+
+SZ = C.c_size_t
+DP = C.POINTER(C.c_double)
+_lib = None
+
+EXPORTS = [
+    "Landau", "Coulomb", "FACG", "FASD", "Coulomb_FA",
+    "glu_gfix_create", "glu_gfix_destroy", "glu_gfix_set_links", "glu_gfix_get_links",
+    "glu_gfix_set_links_device", "glu_gfix_snapshot", "glu_gfix_restore", "glu_gfix_run",
+    "glu_gfix_trace", "glu_gfix_measure", "glu_gfix_theta", "glu_gfix_reunit",
+    "glu_gfix_mixed_split", "glu_gfix_slice_info", "glu_gfix_stage_deriv", "glu_gfix_stage_fa",
+    "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
+    "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
+    "glu_gfix_profile", "glu_gfix_profile_read",
+]
+
+KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge"]
+
+
+class GfixError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GfixError(f"{LIB_PATH} is not built: run __graft_entry__.build() "
+                            "or python -m glu_amd.build")
+        L = C.CDLL(LIB_PATH)
+        P = C.c_void_p
+        sig = {
+            "glu_gfix_create": (C.c_int, [C.POINTER(P), C.POINTER(SZ), C.c_int, C.c_int]),
+            "glu_gfix_destroy": (C.c_int, [P]),
+            "glu_gfix_set_links": (C.c_int, [P, DP]),
+            "glu_gfix_get_links": (C.c_int, [P, DP]),
+            "glu_gfix_set_links_device": (C.c_int, [P, C.c_void_p]),
+            "glu_gfix_snapshot": (C.c_int, [P]),
+            "glu_gfix_restore": (C.c_int, [P]),
+            "glu_gfix_run": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, DP, C.POINTER(SZ)]),
+            "glu_gfix_trace": (C.c_int, [P, DP, SZ, C.POINTER(SZ)]),
+            "glu_gfix_measure": (C.c_int, [P, DP, DP, DP, DP, DP]),
+            "glu_gfix_theta": (C.c_int, [P, DP]),
+            "glu_gfix_reunit": (C.c_int, [P]),
+            "glu_gfix_mixed_split": (C.c_int, [P, C.POINTER(SZ), C.POINTER(SZ)]),
+            "glu_gfix_slice_info": (C.c_int, [P, C.POINTER(SZ), DP]),
+            "glu_gfix_stage_deriv": (C.c_int, [P, DP, DP]),
+            "glu_gfix_stage_fa": (C.c_int, [P, DP, DP]),
+            "glu_gfix_stage_line_search": (C.c_int, [P, DP, DP, DP]),
+            "glu_gfix_stage_egauge": (C.c_int, [P, DP, C.c_double]),
+            "glu_gfix_footprint": (SZ, [C.POINTER(SZ), C.c_int, C.c_int]),
+            "glu_gfix_last_error": (C.c_char_p, []),
+            "glu_gfix_heatbath": (C.c_int, [P, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int]),
+            "glu_gfix_random_transform": (C.c_int, [P, C.c_uint64]),
+            "glu_gfix_profile": (C.c_int, [P, C.c_int]),
+            "glu_gfix_profile_read": (C.c_int, [P, C.c_int, C.POINTER(SZ), DP, DP]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise GfixError(f"{what} failed ({rc}): {lib().glu_gfix_last_error().decode()}")
+
+
+def _dp(a: np.ndarray):
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(DP)
+
+
+class Gfix:
+    """One lattice's device state (wraps glu_gfix_ctx)."""
+
+    def __init__(self, dims, gftype: int = LANDAU, precision: int = FP64):
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        self.gftype = gftype
+        self._h = C.c_void_p()
+        d = (SZ * 4)(*self.dims)
+        _check(lib().glu_gfix_create(C.byref(self._h), d, gftype, precision), "glu_gfix_create")
+
+    def close(self):
+        if self._h:
+            lib().glu_gfix_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_links(self, U: np.ndarray):
+        U = np.ascontiguousarray(U, np.float64)
+        assert U.size == self.V * 72
+        _check(lib().glu_gfix_set_links(self._h, _dp(U)), "set_links")
+
+    def set_links_device(self, ptr: int):
+        _check(lib().glu_gfix_set_links_device(self._h, C.c_void_p(ptr)), "set_links_device")
+
+    def get_links(self) -> np.ndarray:
+        U = np.empty((self.V, 4, 18))
+        _check(lib().glu_gfix_get_links(self._h, _dp(U)), "get_links")
+        return U
+
+    def snapshot(self):
+        _check(lib().glu_gfix_snapshot(self._h), "snapshot")
+
+    def restore(self):
+        _check(lib().glu_gfix_restore(self._h), "restore")
+
+    def run(self, method=CG, gf_tune=0.08, accuracy=1e-14, max_iters=1000):
+        th = C.c_double()
+        it = SZ()
+        _check(lib().glu_gfix_run(self._h, method, gf_tune, accuracy, max_iters,
+                                  C.byref(th), C.byref(it)), "run")
+        return int(it.value), th.value
+
+    def trace(self, cap=16384) -> np.ndarray:
+        rows = np.zeros((cap, 3))
+        n = SZ()
+        _check(lib().glu_gfix_trace(self._h, _dp(rows), cap, C.byref(n)), "trace")
+        return rows[: n.value].copy()
+
+    def measure(self):
+        v = [C.c_double() for _ in range(5)]
+        _check(lib().glu_gfix_measure(self._h, *[C.byref(x) for x in v]), "measure")
+        return dict(zip(("plaquette", "link_trace", "link_max", "functional", "tlink"),
+                        (x.value for x in v)))
+
+    def theta(self) -> float:
+        t = C.c_double()
+        _check(lib().glu_gfix_theta(self._h, C.byref(t)), "theta")
+        return t.value
+
+    def reunit(self):
+        _check(lib().glu_gfix_reunit(self._h), "reunit")
+
+    def mixed_split(self):
+        a, b = SZ(), SZ()
+        _check(lib().glu_gfix_mixed_split(self._h, C.byref(a), C.byref(b)), "mixed_split")
+        return int(a.value), int(b.value)
+
+    def slice_info(self):
+        T = self.dims[3]
+        its = (SZ * T)()
+        th = np.zeros(T)
+        _check(lib().glu_gfix_slice_info(self._h, its, _dp(th)), "slice_info")
+        return list(its), th
+
+    def heatbath(self, beta=6.0, sweeps=200, nor=4, seed=1, cold_start=True):
+        _check(lib().glu_gfix_heatbath(self._h, beta, sweeps, nor, seed, int(cold_start)),
+               "heatbath")
+
+    def random_transform(self, seed=1):
+        _check(lib().glu_gfix_random_transform(self._h, seed), "random_transform")
+
+    def profile(self, enable=True):
+        _check(lib().glu_gfix_profile(self._h, int(enable)), "profile")
+
+    def profile_read(self):
+        """{(kernel, 'fp64'|'fp32'): (launches, mean_ms, total_ms)}"""
+        out = {}
+        for k in range(2 * len(KERNELS)):
+            n, mean, tot = SZ(), C.c_double(), C.c_double()
+            _check(lib().glu_gfix_profile_read(self._h, k, C.byref(n), C.byref(mean), C.byref(tot)),
+                   "profile_read")
+            if n.value:
+                out[(KERNELS[k % len(KERNELS)], "fp32" if k >= len(KERNELS) else "fp64")] = (
+                    int(n.value), mean.value, tot.value)
+        return out
+
+    # stage-level entry points
+    def stage_deriv(self):
+        F = np.empty((4, self.V), np.complex128)
+        t = C.c_double()
+        _check(lib().glu_gfix_stage_deriv(self._h, _dp(F), C.byref(t)), "stage_deriv")
+        return t.value, F
+
+    def stage_fa(self, F):
+        F = np.ascontiguousarray(F, np.complex128)
+        out = np.empty_like(F)
+        _check(lib().glu_gfix_stage_fa(self._h, _dp(F), _dp(out)), "stage_fa")
+        return out
+
+    def stage_line_search(self, F):
+        F = np.ascontiguousarray(F, np.complex128)
+        pr = np.zeros(3)
+        a = C.c_double()
+        _check(lib().glu_gfix_stage_line_search(self._h, _dp(F), _dp(pr), C.byref(a)),
+               "stage_line_search")
+        return pr, a.value
+
+    def stage_egauge(self, F, alpha):
+        F = np.ascontiguousarray(F, np.complex128)
+        _check(lib().glu_gfix_stage_egauge(self._h, _dp(F), alpha), "stage_egauge")
+
+
+def footprint(dims, gftype=LANDAU, precision=FP64) -> int:
+    return int(lib().glu_gfix_footprint((SZ * 4)(*dims), gftype, precision))

@@ -1,0 +1,189 @@
+// fft_kernels.hip -- LDS-staged mixed-radix (2, 3, 4) complex FFT along
+// one lattice axis, batched over every pencil of the 4 packed fields, with
+// the p^2 preconditioner fused into the last forward axis.
+//
+// This replaces the FFTW transforms of Gfix/FACG.c:77-95 (Landau, 4D) and
+// Gfix/CFACG.c:70-89 (Coulomb, 3D per time slice): forward transform,
+// multiply by psq = MAX / p^2 (Gfix/Landau.c:278-284), backward transform,
+// both unnormalised.
+//
+// A workgroup stages P pencils of length N in LDS (N*P <= 2048 complex),
+// runs a Stockham auto-sort radix-4/2/3 sequence on them, and writes them
+// back.  Pencils along x are contiguous rows; along y, z, t consecutive
+// pencils are consecutive addresses, so loads and stores are coalesced in
+// both cases.  Twiddles come from a per-axis table computed on the host.
+#include "gfix_kernels.h"
+
+namespace gfix {
+
+template <typename T> __device__ __forceinline__ cx<T> tw_at(const double2 *tw, int idx, int sign) {
+  const double2 w = tw[idx];
+  return mk<T>((T)w.x, sign < 0 ? (T)w.y : (T)-w.y);
+}
+
+template <typename T, int R> __device__ __forceinline__ void dft_small(cx<T> (&v)[R], int sign) {
+  if constexpr (R == 2) {
+    const cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    const cx<T> a = cadd(v[0], v[2]), b = csub(v[0], v[2]);
+    const cx<T> c = cadd(v[1], v[3]), d = csub(v[1], v[3]);
+    // forward: y1 = b - i d, y3 = b + i d ; backward swaps
+    const cx<T> id = sign < 0 ? mk<T>(d.im, -d.re) : mk<T>(-d.im, d.re);
+    v[0] = cadd(a, c);
+    v[2] = csub(a, c);
+    v[1] = cadd(b, id);
+    v[3] = csub(b, id);
+  } else if constexpr (R == 3) {
+    const T c1 = T(-0.5), s1 = T(sign) * T(0.86602540378443864676);
+    const cx<T> a = v[0], b = v[1], c = v[2];
+    const cx<T> sbc = cadd(b, c), dbc = csub(b, c);
+    v[0] = cadd(a, sbc);
+    const cx<T> m = mk<T>(a.re + c1 * sbc.re, a.im + c1 * sbc.im);
+    // i s1 (b - c)
+    const cx<T> r = mk<T>(-s1 * dbc.im, s1 * dbc.re);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+}
+
+constexpr int kFftThreads = 256;
+constexpr int kFftMaxItems = 4;
+
+// one Stockham stage of radix R on P pencils of length N held in LDS
+template <typename T, int R>
+__device__ __forceinline__ void fft_stage(vec2<T> *lds, int ldP, int N, int P, int Ns, const double2 *tw,
+                                          int sign) {
+  const int nbf = N / R;
+  const int total = nbf * P;
+  const int tstep = N / (Ns * R);
+  cx<T> v[kFftMaxItems][R];
+#pragma unroll
+  for (int it = 0; it < kFftMaxItems; it++) {
+    const int w = threadIdx.x + it * kFftThreads;
+    if (w < total) {
+      const int j = w % P, b = w / P, k = b % Ns;
+#pragma unroll
+      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * ldP + j);
+      if (Ns > 1) {
+#pragma unroll
+        for (int q = 1; q < R; q++) v[it][q] = cmul(v[it][q], tw_at<T>(tw, (q * k * tstep) % N, sign));
+      }
+      dft_small<T, R>(v[it], sign);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < kFftMaxItems; it++) {
+    const int w = threadIdx.x + it * kFftThreads;
+    if (w < total) {
+      const int j = w % P, b = w / P, k = b % Ns;
+      const int out = (b / Ns) * Ns * R + k;
+#pragma unroll
+      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * ldP + j, v[it][q]);
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void fft_run(vec2<T> *lds, int ldP, const FftAxis &ax, int sign) {
+  int Ns = 1;
+  for (int s = 0; s < ax.nst; s++) {
+    const int R = ax.rad[s];
+    if (R == 4)
+      fft_stage<T, 4>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
+    else if (R == 2)
+      fft_stage<T, 2>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
+    else
+      fft_stage<T, 3>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
+    Ns *= R;
+  }
+}
+
+// mode: 0 forward, 1 backward, 2 forward * psq * backward
+// blockIdx.z selects a batch slice of slice_len elements (Coulomb: one time
+// slice); slices with slice_done[z] set are skipped
+template <typename T>
+__global__ __launch_bounds__(kFftThreads) void k_fft_axis(vec2<T> *__restrict__ F, long long fstride, FftAxis ax,
+                                                         int mode, PsqInfo ps, const Ctrl *__restrict__ ctrl,
+                                                         long long slice_len, const int *__restrict__ slice_done) {
+  if (ctrl && ctrl->done) return;
+  if (slice_done && slice_done[blockIdx.z]) return;
+  extern __shared__ unsigned char fft_smem[];
+  vec2<T> *lds = reinterpret_cast<vec2<T> *>(fft_smem);
+  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len;
+  const long long p0 = (long long)blockIdx.x * ax.P;
+  const int ldP = ax.P + 1;
+  const int tot = ax.N * ax.P;
+  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
+    int n, j;
+    if (ax.S == 1) {
+      j = idx / ax.N;
+      n = idx % ax.N;
+    } else {
+      j = idx % ax.P;
+      n = idx / ax.P;
+    }
+    const long long p = p0 + j;
+    if (p < ax.npencil) {
+      const long long gi = (p % ax.S) + (p / ax.S) * ax.S * ax.N + (long long)n * ax.S;
+      lds[n * ldP + j] = base[gi];
+    }
+  }
+  __syncthreads();
+  fft_run<T>(lds, ldP, ax, mode == 1 ? +1 : -1);
+  if (mode == 2) {
+    for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
+      const int j = idx % ax.P, n = idx / ax.P;
+      const long long p = p0 + j;
+      if (p < ax.npencil) {
+        long long gi = (p % ax.S) + (p / ax.S) * ax.S * ax.N + (long long)n * ax.S;
+        // p^2 with the zero mode set to 1: Geometry/geometry.c:170-186
+        double kcos = 0.0;
+        for (int mu = 0; mu < ps.D; mu++) {
+          const int xm = (int)(gi % ps.L[mu]);
+          gi /= ps.L[mu];
+          kcos += ps.cosv[ps.off[mu] + xm];
+        }
+        const double p2 = (kcos == (double)ps.D) ? 1.0 : 2.0 * ((double)ps.D - kcos);
+        const T f = (T)(ps.max / p2);
+        cx<T> z = ld<T>(lds + n * ldP + j);
+        st<T>(lds + n * ldP + j, mk<T>(z.re * f, z.im * f));
+      }
+    }
+    __syncthreads();
+    fft_run<T>(lds, ldP, ax, +1);
+  }
+  for (int idx = threadIdx.x; idx < tot; idx += kFftThreads) {
+    int n, j;
+    if (ax.S == 1) {
+      j = idx / ax.N;
+      n = idx % ax.N;
+    } else {
+      j = idx % ax.P;
+      n = idx / ax.P;
+    }
+    const long long p = p0 + j;
+    if (p < ax.npencil) {
+      const long long gi = (p % ax.S) + (p / ax.S) * ax.S * ax.N + (long long)n * ax.S;
+      base[gi] = lds[n * ldP + j];
+    }
+  }
+}
+
+template <typename T>
+void launch_fft_axis(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+  const unsigned nb = (unsigned)((ax.npencil + ax.P - 1) / ax.P);
+  const size_t sm = (size_t)ax.N * (ax.P + 1) * sizeof(vec2<T>);
+  hipLaunchKernelGGL(k_fft_axis<T>, dim3(nb, nfield, nslice), dim3(kFftThreads), sm, s, (vec2<T> *)F, fstride, ax,
+                     mode, ps, ctrl, slice_len, slice_done);
+}
+template void launch_fft_axis<double>(void *, long long, int, const FftAxis &, int, const PsqInfo &, const Ctrl *,
+                                      hipStream_t, int, long long, const int *);
+template void launch_fft_axis<float>(void *, long long, int, const FftAxis &, int, const PsqInfo &, const Ctrl *,
+                                     hipStream_t, int, long long, const int *);
+
+}  // namespace gfix

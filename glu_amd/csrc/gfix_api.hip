@@ -1,0 +1,679 @@
+// gfix_api.hip -- host side of the plain-pointer C ABI: device contexts,
+// the FACG / FASD iteration drivers (Gfix/FACG.c:181-446) with device-side
+// scalars, and the mixed-precision FP32 -> FP64 schedule.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glu_gfix.h"
+#include "gfix_ctx.h"
+
+using namespace gfix;
+
+static thread_local std::string g_err;
+
+int gfix_fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+extern "C" const char *glu_gfix_last_error(void) { return g_err.c_str(); }
+
+namespace {
+
+constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-455
+constexpr double kSwitchTheta = 1e-7;  // mixed precision: FP32 until theta < 1e-7
+constexpr long long kBatch = 8;        // iterations enqueued between host checks
+
+int factor_radices(int N, int *rad) {
+  int n = 0;
+  while (N % 4 == 0) {
+    rad[n++] = 4;
+    N /= 4;
+  }
+  if (N % 2 == 0) {
+    rad[n++] = 2;
+    N /= 2;
+  }
+  while (N % 3 == 0) {
+    rad[n++] = 3;
+    N /= 3;
+  }
+  return N == 1 ? n : -1;
+}
+
+size_t pow2_floor(size_t x) {
+  size_t p = 1;
+  while (p * 2 <= x) p *= 2;
+  return p;
+}
+
+}  // namespace
+
+size_t gfix_footprint(const size_t dims[4], int gftype, int precision) {
+  const size_t V = dims[0] * dims[1] * dims[2] * dims[3];
+  size_t b = V * 4 * 6 * sizeof(double2);     // FP64 links
+  b += 3 * V * 4 * sizeof(double2);           // in, sn, in_old
+  if (precision == GLU_GFIX_MIXED)
+    b += V * 4 * 6 * sizeof(float2) + V * 6 * sizeof(double2) + 3 * V * 4 * sizeof(float2);
+  if (gftype == GLU_GFIX_COULOMB) b += 2 * V * 6 * sizeof(double2);  // slice transforms + scratch
+  b += ((V + kBlock - 1) / kBlock) * 4 * sizeof(double) + (64 << 10);
+  return b;
+}
+
+extern "C" size_t glu_gfix_footprint(const size_t dims[4], int gftype, int precision) {
+  return gfix_footprint(dims, gftype, precision);
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gftype, int precision) {
+  if (!out || !dims) return gfix_fail(-1, "glu_gfix_create: null argument");
+  *out = nullptr;
+  if (gftype != GLU_GFIX_LANDAU && gftype != GLU_GFIX_COULOMB) return gfix_fail(-1, "unknown gftype %d", gftype);
+  if (precision != GLU_GFIX_FP64 && precision != GLU_GFIX_MIXED)
+    return gfix_fail(-1, "unknown precision %d", precision);
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return gfix_fail(-3, "no HIP device: the gauge-fixing library has no CPU path");
+  glu_gfix_ctx *c = new glu_gfix_ctx();
+  for (int mu = 0; mu < 4; mu++) {
+    if (dims[mu] < 2 || dims[mu] > 4096) {
+      delete c;
+      return gfix_fail(-1, "lattice extent %zu out of range", dims[mu]);
+    }
+    c->dims[mu] = dims[mu];
+    c->g.L[mu] = (int)dims[mu];
+  }
+  c->g.S[0] = 1;
+  for (int mu = 1; mu < 4; mu++) c->g.S[mu] = c->g.S[mu - 1] * (long long)dims[mu - 1];
+  c->g.V = c->g.S[3] * (long long)dims[3];
+  c->g.Lcu = c->g.S[3];
+  c->gftype = gftype;
+  c->precision = precision;
+  const long long V = c->g.V;
+  c->nblk = (V + kBlock - 1) / kBlock;
+  int rc = 0;
+#define CK(x)                                                                  \
+  do {                                                                         \
+    hipError_t e_ = (x);                                                       \
+    if (e_ != hipSuccess) {                                                    \
+      rc = gfix_fail(-2, "%s failed: %s", #x, hipGetErrorString(e_));          \
+      goto bad;                                                                \
+    }                                                                          \
+  } while (0)
+  {
+    CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
+    CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
+    CK(hipMalloc(&c->sn, (size_t)V * 4 * sizeof(double2)));
+    CK(hipMalloc(&c->old, (size_t)V * 4 * sizeof(double2)));
+    CK(hipMalloc(&c->part, (size_t)c->nblk * 4 * sizeof(double) + 4096 * sizeof(double)));
+    CK(hipMalloc(&c->ctrl, sizeof(Ctrl)));
+    CK(hipHostMalloc(&c->hctrl, sizeof(Ctrl), hipHostMallocDefault));
+    CK(hipMalloc(&c->trace, (size_t)c->trace_cap * 3 * sizeof(double)));
+    if (precision == GLU_GFIX_MIXED) {
+      CK(hipMalloc(&c->U32, (size_t)V * 24 * sizeof(float2)));
+      CK(hipMalloc(&c->G64, (size_t)V * 6 * sizeof(double2)));
+      CK(hipMalloc(&c->F32, (size_t)V * 4 * sizeof(float2)));
+      CK(hipMalloc(&c->sn32, (size_t)V * 4 * sizeof(float2)));
+      CK(hipMalloc(&c->old32, (size_t)V * 4 * sizeof(float2)));
+      if (const char *e = getenv("GLU_GFIX_SWITCH_THETA")) c->switch_theta = atof(e);
+    }
+    // FFT axes, twiddles and the p^2 cos tables
+    const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
+    std::vector<double> cosv;
+    for (int mu = 0; mu < 4; mu++) {
+      const int N = (int)dims[mu];
+      c->ps.off[mu] = (int)cosv.size();
+      c->ps.L[mu] = N;
+      const double twd = 6.283185307179586 / (double)N;  // Latt.twiddles, Geometry/init.c:69-71
+      for (int n = 0; n < N; n++) cosv.push_back(cos(n * twd));
+      if (mu >= D) continue;
+      FftAxis ax{};
+      ax.N = N;
+      ax.nst = factor_radices(N, ax.rad);
+      if (ax.nst < 0) {
+        rc = gfix_fail(-1, "lattice extent %d: only radices 2 and 3 are supported", N);
+        goto bad;
+      }
+      ax.S = c->g.S[mu];
+      ax.npencil = (D == 4 ? V : c->g.Lcu) / N;  // per transform (Coulomb: per slice)
+      ax.P = (int)std::min<size_t>(512, pow2_floor(2048 / (size_t)N));
+      std::vector<double2> tw(N);
+      for (int k = 0; k < N; k++) {
+        const double a = 6.283185307179586 * (double)k / (double)N;
+        tw[k] = make_double2(cos(a), -sin(a));
+      }
+      CK(hipMalloc(&c->tw[mu], N * sizeof(double2)));
+      CK(hipMemcpy(c->tw[mu], tw.data(), N * sizeof(double2), hipMemcpyHostToDevice));
+      ax.tw = c->tw[mu];
+      c->ax[mu] = ax;
+    }
+    CK(hipMalloc(&c->cosv, cosv.size() * sizeof(double)));
+    CK(hipMemcpy(c->cosv, cosv.data(), cosv.size() * sizeof(double), hipMemcpyHostToDevice));
+    c->ps.D = D;
+    c->ps.cosv = c->cosv;
+    c->ps.max = D == 4 ? 4.0 * 4 / (double)V : 4.0 * 3 / (double)c->g.Lcu;
+    if (gftype == GLU_GFIX_COULOMB) {
+      rc = coulomb_alloc(c);
+      if (rc) goto bad;
+    }
+  }
+#undef CK
+  *out = c;
+  return 0;
+bad:
+  glu_gfix_destroy(c);
+  return rc;
+}
+
+extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
+  if (!c) return 0;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void *bufs[] = {c->U64, c->U32, c->G64, c->Usnap, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
+                  c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->F32, c->sn32, c->old32};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  coulomb_free(c);
+  prof_harvest(c);
+  for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
+  if (c->hctrl) hipHostFree(c->hctrl);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// links in / out
+// ---------------------------------------------------------------------------
+static const long long kChunk = 1 << 17;  // sites per staging transfer
+
+static int ensure_stage(glu_gfix_ctx *c) {
+  if (c->stage) return 0;
+  if (hipMalloc(&c->stage, (size_t)kChunk * 72 * sizeof(double)) != hipSuccess)
+    return gfix_fail(-2, "staging allocation failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_set_links(glu_gfix_ctx *c, const double *links) {
+  if (!c || !links) return gfix_fail(-1, "glu_gfix_set_links: null argument");
+  if (int rc = ensure_stage(c)) return rc;
+  const long long V = c->g.V;
+  for (long long s0 = 0; s0 < V; s0 += kChunk) {
+    const long long n = std::min(kChunk, V - s0);
+    if (hipMemcpyAsync(c->stage, links + s0 * 72, (size_t)n * 72 * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess)
+      return gfix_fail(-2, "link upload failed");
+    launch_pack<double>((const double *)c->stage, c->U64, V, 4, s0, n, c->stream);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "link upload: %s", "sync failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_set_links_device(glu_gfix_ctx *c, const double *links_dev) {
+  if (!c || !links_dev) return gfix_fail(-1, "glu_gfix_set_links_device: null argument");
+  launch_pack<double>(links_dev, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "pack failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_get_links(glu_gfix_ctx *c, double *links) {
+  if (!c || !links) return gfix_fail(-1, "glu_gfix_get_links: null argument");
+  if (int rc = ensure_stage(c)) return rc;
+  const long long V = c->g.V;
+  for (long long s0 = 0; s0 < V; s0 += kChunk) {
+    const long long n = std::min(kChunk, V - s0);
+    launch_unpack<double>(c->U64, (double *)c->stage, V, 4, s0, n, c->stream);
+    if (hipMemcpyAsync(links + s0 * 72, c->stage, (size_t)n * 72 * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess)
+      return gfix_fail(-2, "link download failed");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "link download sync failed");
+  }
+  return 0;
+}
+
+extern "C" int glu_gfix_snapshot(glu_gfix_ctx *c) {
+  if (!c) return gfix_fail(-1, "null context");
+  const size_t bytes = (size_t)c->g.V * 24 * sizeof(double2);
+  if (!c->Usnap && hipMalloc(&c->Usnap, bytes) != hipSuccess) return gfix_fail(-2, "snapshot allocation failed");
+  if (hipMemcpyAsync(c->Usnap, c->U64, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "snapshot copy failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_restore(glu_gfix_ctx *c) {
+  if (!c || !c->Usnap) return gfix_fail(-1, "no snapshot to restore");
+  const size_t bytes = (size_t)c->g.V * 24 * sizeof(double2);
+  if (hipMemcpyAsync(c->U64, c->Usnap, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    return gfix_fail(-2, "restore copy failed");
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// iteration building blocks
+// ---------------------------------------------------------------------------
+template <typename T> static int pk(int id) { return id + (sizeof(T) == 4 ? PK_COUNT : 0); }
+
+// forward transform, p^2 preconditioner, backward transform of the 4 packed
+// fields: over the whole lattice (Landau) or per time slice (Coulomb)
+template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, const int *slice_done) {
+  const int D = c->ps.D;
+  const long long V = c->g.V;
+  const int ns = D == 4 ? 1 : c->g.L[3];
+  const long long sl = D == 4 ? 0 : c->g.Lcu;
+  for (int a = 0; a < D - 1; a++)
+    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 0, c->ps, ctrl, c->stream, ns, sl, slice_done));
+  GFIX_PROF(c, pk<T>(PK_FFT),
+            launch_fft_axis<T>(F, V, 4, c->ax[D - 1], 2, c->ps, ctrl, c->stream, ns, sl, slice_done));
+  for (int a = D - 2; a >= 0; a--)
+    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 1, c->ps, ctrl, c->stream, ns, sl, slice_done));
+}
+
+hipEvent_t prof_event(glu_gfix_ctx *c) {
+  if (!c->prof_pool.empty()) {
+    hipEvent_t e = c->prof_pool.back();
+    c->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hipEventCreate(&e);
+  return e;
+}
+
+// collect the durations of recorded launches (call after a stream sync)
+void prof_harvest(glu_gfix_ctx *c) {
+  for (const ProfPending &p : c->prof_pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) c->prof_ms[p.id].push_back(ms);
+    c->prof_pool.push_back(p.a);
+    c->prof_pool.push_back(p.b);
+  }
+  c->prof_pending.clear();
+}
+template void enq_fa<double>(glu_gfix_ctx *, void *, const Ctrl *, const int *);
+template void enq_fa<float>(glu_gfix_ctx *, void *, const Ctrl *, const int *);
+
+static double landau_norm(const glu_gfix_ctx *c) { return 1.0 / (3.0 * (double)c->g.V); }
+
+// the buffers one precision phase works on
+struct Bufs {
+  void *U, *G, *F, *sn, *old;
+};
+template <typename T> static Bufs bufs_for(glu_gfix_ctx *c) {
+  if (sizeof(T) == 4) return {c->U32, c->G64, c->F32, c->sn32, c->old32};
+  return {c->U64, nullptr, c->F, c->sn, c->old};
+}
+
+// steepest-descent start of a FACG segment (Gfix/FACG.c:249-265)
+template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
+  hipStream_t s = c->stream;
+  launch_begin_segment(c->ctrl, 1, s);
+  launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s);
+  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
+  enq_fa<T>(c, b.F, c->ctrl, nullptr);
+  launch_line_search<T>(b.U, b.F, c->part, c->g, c->ctrl, 1, s);
+  launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 1, 0.15, 0.30, s);
+  launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 0, s);
+  launch_copy3<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s);
+  launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
+  launch_fin_sd(c->part, c->nblk, c->ctrl, s);
+}
+
+// one FACG iteration (Gfix/FACG.c:272-334)
+template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
+  hipStream_t s = c->stream;
+  GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
+  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
+  enq_fa<T>(c, b.F, c->ctrl, nullptr);
+  GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
+  launch_fin_pr(c->part, c->nblk, c->ctrl, s);
+  GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s));
+  GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, c->g, c->ctrl, 0, s));
+  launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
+  GFIX_PROF(c, pk<T>(PK_EGAUGE), launch_egauge<T>(b.U, b.sn, b.G, c->g, c->ctrl, 0, s));
+  launch_iter_end(c->ctrl, c->trace, 0, s);
+}
+
+// one FASD iteration (Gfix/FACG.c:202-235, fixed step GF_TUNE)
+template <typename T> static void enq_sd_iter(glu_gfix_ctx *c, const Bufs &b) {
+  hipStream_t s = c->stream;
+  launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s);
+  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
+  enq_fa<T>(c, b.F, c->ctrl, nullptr);
+  launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 1, s);
+  launch_iter_end(c->ctrl, c->trace, 1, s);
+}
+
+static int read_ctrl(glu_gfix_ctx *c) {
+  if (hipMemcpyAsync(c->hctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "control block read failed: %s", hipGetErrorString(hipGetLastError()));
+  if (c->prof) prof_harvest(c);
+  return 0;
+}
+static int write_ctrl(glu_gfix_ctx *c) {
+  if (hipMemcpyAsync(c->ctrl, c->hctrl, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "control block write failed");
+  return 0;
+}
+
+// theta_test_lin = 2 * (FA theta) on the links U (Gfix/gftests.c:215-250)
+template <typename T> static int theta_lin(glu_gfix_ctx *c, void *U, double *out) {
+  launch_deriv<T>(U, c->F, c->part, c->g, nullptr, c->stream);
+  std::vector<double> h(c->nblk);
+  if (hipMemcpyAsync(h.data(), c->part, c->nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "theta readback failed");
+  double s = 0.0;
+  for (double v : h) s += v;
+  *out = 2.0 * s * landau_norm(c);
+  return 0;
+}
+
+// FACG() / FASD() of Gfix/FACG.c:341-446 on the phase buffers of type T.
+// switch_theta > 0: stop early once theta < switch_theta (mixed precision).
+// resume: continue the CG state already in the control block and fields
+// (no steepest-descent start) -- used after the FP32 -> FP64 switch.
+// Returns GLU's iteration count; *switched tells whether the stop was the
+// precision switch.
+template <typename T>
+static int landau_run(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double switch_theta,
+                      int resume, double *theta, size_t *iters, int *switched, long long *done_iters) {
+  const Bufs b = bufs_for<T>(c);
+  Ctrl &h = *c->hctrl;
+  if (resume) {
+    if (int rc = read_ctrl(c)) return rc;
+    h.switch_theta = switch_theta;
+    h.switched = 0;
+    h.total = 0;
+  } else {
+    memset(&h, 0, sizeof(Ctrl));
+    h.acc = acc;
+    h.gf_alpha = gf;
+    h.cg_tol = kCgTol;
+    h.switch_theta = switch_theta;
+    h.trace_cap = c->trace_cap;
+    h.trace_n = c->trace_n;
+  }
+  h.max_iters = (long long)max_iters;
+  h.done = 1;
+  if (int rc = write_ctrl(c)) return rc;
+  *switched = 0;
+  *done_iters = 0;
+  size_t out = 0;
+  bool first = true;
+  // GLU repeats continuation segments without bound (Gfix/FACG.c:363-379);
+  // a cap keeps a pathological input from spinning forever
+  for (int segment = 0; segment < 64; segment++) {
+    if (method == GLU_GFIX_CG && !(resume && first))
+      enq_sd_start<T>(c, b);
+    else
+      launch_begin_segment(c->ctrl, 0, c->stream);
+    first = false;
+    for (;;) {
+      for (long long k = 0; k < kBatch; k++) {
+        if (method == GLU_GFIX_CG)
+          enq_cg_iter<T>(c, b);
+        else
+          enq_sd_iter<T>(c, b);
+      }
+      if (int rc = read_ctrl(c)) return rc;
+      if (h.done) break;
+    }
+    *theta = h.theta;
+    const long long loc = h.iters;
+    *done_iters = h.total;
+    c->trace_n = h.trace_n;
+    if (h.switched) {
+      *switched = 1;
+      out = (size_t)loc;
+      break;
+    }
+    if (method == GLU_GFIX_CG) {
+      out = (size_t)loc;
+      if ((size_t)loc >= max_iters) {
+        if (h.theta < 1e3 * acc) {
+          fprintf(stdout, "[GF] Continuation run \n");
+          continue;
+        }
+        out = 123456789;
+      }
+    } else {
+      if ((size_t)loc >= max_iters) {
+        if (h.theta < 1e3 * acc) {
+          fprintf(stdout, "[GF] Continuation run \n");
+          continue;
+        }
+        out = 123456789 + (size_t)loc;
+      } else {
+        out = (size_t)loc;
+      }
+    }
+    break;
+  }
+  *iters = out;
+  return 0;
+}
+
+int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters) {
+  double th0 = 0.0;
+  c->trace_n = 0;
+  c->last_fp32_iters = c->last_fp64_iters = 0;
+  if (int rc = theta_lin<double>(c, c->U64, &th0)) return rc;
+  *theta = th0;
+  if (th0 < acc) {  // Gfix/FACG.c:349-353
+    *iters = 0;
+    return 0;
+  }
+  int sw = 0;
+  long long tot = 0;
+  if (c->precision == GLU_GFIX_MIXED && c->switch_theta > acc) {
+    // FP32 phase on a demoted copy; the site transform accumulates in FP64
+    launch_demote(c->U64, c->U32, c->g.V * 24, c->stream);
+    launch_identity<double>(c->G64, c->g.V, c->stream);
+    size_t it32 = 0;
+    double th32 = 0.0;
+    if (int rc = landau_run<float>(c, method, gf, acc, max_iters, c->switch_theta, 0, &th32, &it32, &sw, &tot))
+      return rc;
+    c->last_fp32_iters = (size_t)tot;
+    // replay the accumulated transform on the FP64 links, carry the CG state
+    launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
+    launch_promote(c->sn32, c->sn, c->g.V * 4, c->stream);
+    launch_promote(c->old32, c->old, c->g.V * 4, c->stream);
+    if (!sw && (it32 >= 123456789 || !(th32 > acc))) {  // finished (or failed) in FP32
+      *iters = it32;
+      *theta = th32;
+      return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : gfix_fail(-2, "sync failed");
+    }
+    size_t it64 = 0;
+    const size_t left = max_iters > (size_t)tot ? max_iters - (size_t)tot : 1;
+    if (int rc = landau_run<double>(c, method, gf, acc, left, 0.0, sw, theta, &it64, &sw, &tot)) return rc;
+    c->last_fp64_iters = (size_t)tot;
+    *iters = it64 >= 123456789 ? it64 : c->last_fp32_iters + c->last_fp64_iters;
+    return 0;
+  }
+  if (int rc = landau_run<double>(c, method, gf, acc, max_iters, 0.0, 0, theta, iters, &sw, &tot)) return rc;
+  c->last_fp64_iters = (size_t)tot;
+  return 0;
+}
+
+extern "C" int glu_gfix_run(glu_gfix_ctx *c, int method, double gf_tune, double accuracy, size_t max_iters,
+                            double *theta, size_t *iters) {
+  if (!c || !theta || !iters) return gfix_fail(-1, "glu_gfix_run: null argument");
+  if (method != GLU_GFIX_CG && method != GLU_GFIX_SD) return gfix_fail(-1, "unknown method %d", method);
+  if (max_iters == 0) return gfix_fail(-1, "max_iters must be positive");
+  if (c->gftype == GLU_GFIX_COULOMB) return coulomb_fix(c, method, gf_tune, accuracy, max_iters, theta, iters);
+  return landau_fix(c, method, gf_tune, accuracy, max_iters, theta, iters);
+}
+
+extern "C" int glu_gfix_trace(glu_gfix_ctx *c, double *rows, size_t cap, size_t *n) {
+  if (!c || !n) return gfix_fail(-1, "glu_gfix_trace: null argument");
+  const size_t m = std::min<size_t>(cap, (size_t)c->trace_n);
+  if (m && hipMemcpy(rows, c->trace, m * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return gfix_fail(-2, "trace readback failed");
+  *n = m;
+  return 0;
+}
+
+extern "C" int glu_gfix_mixed_split(glu_gfix_ctx *c, size_t *a, size_t *b) {
+  if (!c || !a || !b) return gfix_fail(-1, "null argument");
+  *a = c->last_fp32_iters;
+  *b = c->last_fp64_iters;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// observables
+// ---------------------------------------------------------------------------
+extern "C" int glu_gfix_measure(glu_gfix_ctx *c, double *plaq, double *tr, double *mx, double *func, double *tlink) {
+  if (!c) return gfix_fail(-1, "null context");
+  launch_measure(c->U64, c->part, c->g, c->stream);
+  std::vector<double> h(c->nblk * 4);
+  if (hipMemcpyAsync(h.data(), c->part, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "measure readback failed");
+  double p = 0, t = 0, tt = 0, m = -1e300;
+  for (long long b = 0; b < c->nblk; b++) {
+    p += h[4 * b];
+    t += h[4 * b + 1];
+    tt += h[4 * b + 2];
+    m = h[4 * b + 3] > m ? h[4 * b + 3] : m;
+  }
+  const double V = (double)c->g.V;
+  if (plaq) *plaq = 2.0 * p / (3.0 * 4 * 3 * V);
+  if (tr) *tr = t / (4 * 3 * V);
+  if (mx) *mx = m / 3.0;
+  if (func) *func = 1.0 - t / (4 * 3 * V);
+  if (tlink) *tlink = tt / (3 * V);
+  return 0;
+}
+
+extern "C" int glu_gfix_theta(glu_gfix_ctx *c, double *th) {
+  if (!c || !th) return gfix_fail(-1, "null argument");
+  return theta_lin<double>(c, c->U64, th);
+}
+
+extern "C" int glu_gfix_reunit(glu_gfix_ctx *c) {
+  if (!c) return gfix_fail(-1, "null context");
+  launch_reunit(c->U64, c->g.V, 4, c->stream);
+  return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : gfix_fail(-2, "reunit failed");
+}
+
+// ---------------------------------------------------------------------------
+// stage-level entry points (FP64)
+// ---------------------------------------------------------------------------
+static int fields_in(glu_gfix_ctx *c, const double *f, void *dst) {
+  return hipMemcpyAsync(dst, f, (size_t)c->g.V * 4 * sizeof(double2), hipMemcpyHostToDevice, c->stream) ==
+                 hipSuccess
+             ? 0
+             : gfix_fail(-2, "field upload failed");
+}
+static int fields_out(glu_gfix_ctx *c, const void *src, double *f) {
+  if (hipMemcpyAsync(f, src, (size_t)c->g.V * 4 * sizeof(double2), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "field download failed");
+  return 0;
+}
+static int ctrl_live(glu_gfix_ctx *c, double alpha, double theta) {
+  memset(c->hctrl, 0, sizeof(Ctrl));
+  c->hctrl->alpha = alpha;
+  c->hctrl->gf_alpha = alpha;
+  c->hctrl->theta = theta;
+  c->hctrl->cg_tol = kCgTol;
+  return write_ctrl(c);
+}
+
+extern "C" int glu_gfix_stage_deriv(glu_gfix_ctx *c, double *fields, double *theta) {
+  if (!c || !fields || !theta) return gfix_fail(-1, "null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "stage entry points are Landau-only");
+  if (int rc = ctrl_live(c, 0.0, 1.0)) return rc;
+  launch_deriv<double>(c->U64, c->F, c->part, c->g, c->ctrl, c->stream);
+  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), c->stream);
+  if (int rc = fields_out(c, c->F, fields)) return rc;
+  if (int rc = read_ctrl(c)) return rc;
+  *theta = c->hctrl->theta;
+  return 0;
+}
+
+extern "C" int glu_gfix_stage_fa(glu_gfix_ctx *c, const double *fin, double *fout) {
+  if (!c || !fin || !fout) return gfix_fail(-1, "null argument");
+  if (int rc = ctrl_live(c, 0.0, 1.0)) return rc;
+  if (int rc = fields_in(c, fin, c->F)) return rc;
+  enq_fa<double>(c, c->F, c->ctrl, nullptr);
+  return fields_out(c, c->F, fout);
+}
+
+extern "C" int glu_gfix_stage_line_search(glu_gfix_ctx *c, const double *fields, double probes[3], double *alpha) {
+  if (!c || !fields || !probes || !alpha) return gfix_fail(-1, "null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "stage entry points are Landau-only");
+  if (int rc = ctrl_live(c, 0.0, 1.0)) return rc;
+  if (int rc = fields_in(c, fields, c->sn)) return rc;
+  launch_line_search<double>(c->U64, c->sn, c->part, c->g, c->ctrl, 1, c->stream);
+  launch_fin_ls(c->part, c->nblk, c->ctrl, nullptr, 1, 0.15, 0.30, c->stream);
+  if (int rc = read_ctrl(c)) return rc;
+  for (int k = 0; k < 3; k++) probes[k] = c->hctrl->probe[k];
+  *alpha = c->hctrl->alpha;
+  return 0;
+}
+
+extern "C" int glu_gfix_stage_egauge(glu_gfix_ctx *c, const double *fields, double alpha) {
+  if (!c || !fields) return gfix_fail(-1, "null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "stage entry points are Landau-only");
+  if (int rc = ctrl_live(c, alpha, 1.0)) return rc;
+  if (int rc = fields_in(c, fields, c->sn)) return rc;
+  launch_egauge<double>(c->U64, c->sn, nullptr, c->g, c->ctrl, 0, c->stream);
+  return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : gfix_fail(-2, "egauge failed");
+}
+
+// ---------------------------------------------------------------------------
+// per-kernel timing
+// ---------------------------------------------------------------------------
+extern "C" int glu_gfix_profile(glu_gfix_ctx *c, int enable) {
+  if (!c) return gfix_fail(-1, "null context");
+  hipStreamSynchronize(c->stream);
+  prof_harvest(c);
+  c->prof = enable;
+  if (enable)
+    for (auto &v : c->prof_ms) v.clear();
+  return 0;
+}
+
+extern "C" int glu_gfix_profile_read(glu_gfix_ctx *c, int kernel, size_t *launches, double *mean_ms,
+                                     double *total_ms) {
+  if (!c || kernel < 0 || kernel >= 2 * PK_COUNT || !launches || !mean_ms || !total_ms)
+    return gfix_fail(-1, "glu_gfix_profile_read: bad argument");
+  hipStreamSynchronize(c->stream);
+  prof_harvest(c);
+  std::vector<float> v = c->prof_ms[kernel];
+  *launches = 0;
+  *mean_ms = *total_ms = 0.0;
+  if (v.empty()) return 0;
+  // launches issued past convergence return at once; count only live ones
+  std::vector<float> srt = v;
+  std::sort(srt.begin(), srt.end());
+  const float med = srt[srt.size() / 2];
+  double tot = 0.0;
+  size_t n = 0;
+  for (float ms : v)
+    if (ms >= 0.5f * med) {
+      tot += ms;
+      n++;
+    }
+  *launches = n;
+  *total_ms = tot;
+  *mean_ms = n ? tot / n : 0.0;
+  return 0;
+}

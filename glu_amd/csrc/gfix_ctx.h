@@ -1,0 +1,77 @@
+// gfix_ctx.h -- the device context behind glu_gfix_ctx (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "gfix_kernels.h"
+
+#include <vector>
+
+struct CoulombState;
+
+// per-kernel event timing (glu_gfix_profile): kernel classes
+enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_COUNT };
+struct ProfPending {
+  int id;
+  hipEvent_t a, b;
+};
+
+struct glu_gfix_ctx {
+  size_t dims[4] = {0, 0, 0, 0};
+  gfix::Geo g{};
+  int gftype = 0;
+  int precision = 0;
+  hipStream_t stream = nullptr;
+  // links: FP64 column planes (4 dirs x 6 entries x V), FP32 copy and the
+  // FP32 accumulated site transform for the mixed schedule, snapshot
+  void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
+  // FP32 packed fields of the mixed schedule (in, sn, in_old)
+  void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
+  double switch_theta = 1e-7;
+  // packed fields (4 x V complex each): in, sn (search direction), in_old
+  void *F = nullptr, *sn = nullptr, *old = nullptr;
+  double *part = nullptr;  // per-block partial sums
+  long long nblk = 0;
+  gfix::Ctrl *ctrl = nullptr;   // device control block
+  gfix::Ctrl *hctrl = nullptr;  // pinned host mirror
+  double *trace = nullptr;
+  long long trace_cap = 1 << 14;
+  long long trace_n = 0;
+  double2 *tw[4] = {nullptr, nullptr, nullptr, nullptr};
+  double *cosv = nullptr;
+  gfix::FftAxis ax[4]{};
+  gfix::PsqInfo ps{};
+  void *stage = nullptr;  // upload / download staging
+  size_t last_fp32_iters = 0, last_fp64_iters = 0;
+  CoulombState *cs = nullptr;
+  // profiling: id = kernel class + PK_COUNT for the FP32 variants
+  int prof = 0;
+  std::vector<ProfPending> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
+  std::vector<float> prof_ms[2 * PK_COUNT];
+};
+
+// time one launch statement with HIP events on the context stream
+#define GFIX_PROF(c, id, stmt)                                       \
+  do {                                                              \
+    if ((c)->prof) {                                                \
+      ProfPending p_{(id), prof_event(c), prof_event(c)};           \
+      hipEventRecord(p_.a, (c)->stream);                            \
+      stmt;                                                         \
+      hipEventRecord(p_.b, (c)->stream);                            \
+      (c)->prof_pending.push_back(p_);                              \
+    } else {                                                        \
+      stmt;                                                         \
+    }                                                               \
+  } while (0)
+hipEvent_t prof_event(glu_gfix_ctx *c);
+void prof_harvest(glu_gfix_ctx *c);
+
+int gfix_fail(int code, const char *fmt, ...);
+size_t gfix_footprint(const size_t dims[4], int gftype, int precision);
+template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const gfix::Ctrl *ctrl, const int *slice_done);
+
+int coulomb_alloc(glu_gfix_ctx *c);
+void coulomb_free(glu_gfix_ctx *c);
+int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters);
+int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters);

@@ -1,0 +1,99 @@
+// gfix_device.h -- lattice geometry, device control block and reductions
+// shared by the gauge-fixing kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "su3.h"
+
+namespace gfix {
+
+constexpr int kNC = 3;
+constexpr int kND = 4;
+constexpr int kBlock = 256;  // threads per block of the streaming kernels
+constexpr int kWave = 64;
+
+// periodic 4D lattice, x fastest (Geometry/geometry.c:28-36)
+struct Geo {
+  int L[4];
+  long long S[4];  // strides 1, Lx, LxLy, LxLyLz
+  long long V;
+  long long Lcu;
+};
+
+__device__ __forceinline__ void coords(const Geo &g, long long i, int x[4]) {
+  long long r = i;
+  x[0] = (int)(r % g.L[0]);
+  r /= g.L[0];
+  x[1] = (int)(r % g.L[1]);
+  r /= g.L[1];
+  x[2] = (int)(r % g.L[2]);
+  x[3] = (int)(r / g.L[2]);
+}
+__device__ __forceinline__ long long nb_bwd(const Geo &g, long long i, const int x[4], int mu) {
+  return x[mu] > 0 ? i - g.S[mu] : i + (long long)(g.L[mu] - 1) * g.S[mu];
+}
+__device__ __forceinline__ long long nb_fwd(const Geo &g, long long i, const int x[4], int mu) {
+  return x[mu] < g.L[mu] - 1 ? i + g.S[mu] : i - (long long)(g.L[mu] - 1) * g.S[mu];
+}
+
+// Device-resident scalars of one FACG/FASD run.  Every per-iteration kernel
+// reads `done` first and returns when it is set, so a captured batch of
+// iterations becomes a no-op past convergence.
+struct Ctrl {
+  double theta;      // theta_k of the current iteration (GFNORM_LANDAU)
+  double inold;      // Polak-Ribiere denominator
+  double insum;      // <in, in>
+  double sum_conj;   // <in, in - in_old>
+  double beta;
+  double alpha;      // step of the current update
+  double probe[3];   // line-search functionals at 0, PL1, PL2
+  double acc;        // stopping accuracy
+  double gf_alpha;   // GF_TUNE
+  double cg_tol;     // PR -> FR switch (CG_TOL)
+  double switch_theta;  // mixed precision: FP32 -> FP64 switch
+  double stale_probe[3];  // GLU's red[] slots left by the previous segment
+  double stale_insum;
+  double stale_theta;
+  long long iters;   // iterations in the current segment
+  long long max_iters;
+  long long total;   // iterations over all segments (monitoring)
+  int done;          // set after the update of the iteration that stopped
+  int did_ls;        // current iteration used the line search
+  int sd_start;      // currently inside the SD start
+  int switched;      // mixed precision reached switch_theta
+  long long trace_cap;
+  long long trace_n;
+};
+
+// ---------------------------------------------------------------- reduce --
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_down(v, o, kWave);
+  return v;
+}
+
+// Sum NV values over the block; thread 0 writes them to out[0..NV).
+// The order of additions is fixed, so results are reproducible.
+template <int NV>
+__device__ __forceinline__ void block_sum_store(double (&v)[NV], double *out) {
+  __shared__ double sh[NV][kBlock / kWave];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][w] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = 0.0;
+      for (int j = 0; j < (int)(blockDim.x / kWave); j++) s += sh[k][j];
+      out[k] = s;
+    }
+  }
+}
+
+}  // namespace gfix

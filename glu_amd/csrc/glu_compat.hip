@@ -1,0 +1,493 @@
+// glu_compat.hip -- GLU drop-in entry points: Landau(), Coulomb(), FACG(),
+// FASD() and Coulomb_FA() with GLU's prototypes, reading GLU's global
+// `Latt` and overwriting the links of `lat` in place.
+//
+// The host side mirrors GLU's drivers (Gfix/Landau.c:261-374,
+// Gfix/Coulomb.c:33-100, Gfix/FACG.c:341-446): the same theta test,
+// continuation, random restarts and [GF] report lines.  The work itself
+// runs on the device through the glu_gfix_* context API.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/glu_gfix.h"
+#include "gfix_ctx.h"
+
+// GLU defines `Latt` (Run/Mainfile.c); standalone users get this weak one.
+extern "C" {
+__attribute__((weak)) struct latt_info Latt;
+}
+
+namespace {
+
+using cd = std::complex<double>;
+
+size_t volume() { return Latt.dims[0] * Latt.dims[1] * Latt.dims[2] * Latt.dims[3]; }
+
+int precision_from_env() {
+  const char *p = getenv("GLU_GFIX_PRECISION");
+  return (p && (!strcmp(p, "mixed") || !strcmp(p, "MIXED"))) ? GLU_GFIX_MIXED : GLU_GFIX_FP64;
+}
+
+// one cached device context per (dims, gftype, precision)
+glu_gfix_ctx *g_ctx = nullptr;
+size_t g_dims[4];
+int g_type = -1, g_prec = -1;
+
+glu_gfix_ctx *get_ctx(int gftype) {
+  const int prec = precision_from_env();
+  if (g_ctx && g_type == gftype && g_prec == prec && !memcmp(g_dims, Latt.dims, sizeof(g_dims))) return g_ctx;
+  if (g_ctx) glu_gfix_destroy(g_ctx);
+  g_ctx = nullptr;
+  if (glu_gfix_create(&g_ctx, Latt.dims, gftype, prec) != 0) {
+    fprintf(stderr, "[GF] device context creation failed: %s\n", glu_gfix_last_error());
+    g_ctx = nullptr;
+    return nullptr;
+  }
+  memcpy(g_dims, Latt.dims, sizeof(g_dims));
+  g_type = gftype;
+  g_prec = prec;
+  return g_ctx;
+}
+
+void pack(const struct site *lat, std::vector<double> &h) {
+  const size_t V = volume();
+  h.resize(V * 72);
+  for (size_t i = 0; i < V; i++)
+    for (int mu = 0; mu < 4; mu++) memcpy(&h[(i * 4 + mu) * 18], lat[i].O[mu], 18 * sizeof(double));
+}
+void unpack(struct site *lat, const std::vector<double> &h) {
+  const size_t V = volume();
+  for (size_t i = 0; i < V; i++)
+    for (int mu = 0; mu < 4; mu++) memcpy(lat[i].O[mu], &h[(i * 4 + mu) * 18], 18 * sizeof(double));
+}
+
+// ------------------------------------------------------------------------
+// GLU's parallel MWC_4096 generator and the random gauge transform used by
+// the Landau restarts (Utils/par_rng.c:43-186, Utils/par_MWC_4096.c:33-112,
+// Geometry/random_config.c:108-120).  One stream per GLU thread, seeded
+// SEED + thread (SEED 0: /dev/urandom), warmed up by 10000 draws, each
+// thread drawing its static block of sites.
+// ------------------------------------------------------------------------
+struct Mwc {
+  std::vector<uint32_t> x, carry;
+  bool init = false;
+} g_rng;
+
+uint32_t mwc_next(int t) {
+  const uint64_t p = 18782ULL * (uint64_t)g_rng.x[t] + g_rng.carry[t];
+  g_rng.carry[t] = (uint32_t)(p >> 32);
+  uint32_t v = (uint32_t)p + g_rng.carry[t];
+  if (v < g_rng.carry[t]) {
+    v++;
+    g_rng.carry[t]++;
+  }
+  g_rng.x[t] = 0xfffffffeu - v;
+  return g_rng.x[t];
+}
+double mwc_dbl(int t) { return (double)(mwc_next(t) * 2.3283064365386963e-10); }
+
+void rng_init() {
+  if (g_rng.init) return;
+  const int nth = Latt.Nthreads ? (int)Latt.Nthreads : 1;
+  uint32_t seed = Latt.Seed[0];
+  if (seed == 0) {
+    FILE *f = fopen("/dev/urandom", "r");
+    if (f) {
+      if (fread(&seed, sizeof(seed), 1, f) != 1) seed = 1;
+      fclose(f);
+    }
+  }
+  g_rng.x.resize(nth);
+  g_rng.carry.resize(nth);
+  std::vector<uint32_t> tab(4096);
+  for (int t = 0; t < nth; t++) {
+    tab[0] = seed + (uint32_t)t;
+    for (uint32_t j = 1; j < 4096; j++) tab[j] = (uint32_t)(1812433253UL * (tab[j - 1] ^ (tab[j - 1] >> 30)) + j);
+    g_rng.x[t] = tab[0];
+    g_rng.carry[t] = tab[4095] % 809430660UL;
+    for (int j = 0; j < 10000; j++) mwc_dbl(t);
+  }
+  g_rng.init = true;
+}
+
+cd polar_box(int t) {
+  for (;;) {
+    const double u = 2.0 * mwc_dbl(t) - 1.0, v = 2.0 * mwc_dbl(t) - 1.0;
+    const double s = u * u + v * v;
+    if (s < 1.0) return sqrt(-log(s) / s) * cd(u, v);
+  }
+}
+
+void reunit_host(cd *U) {
+  double n = std::norm(U[0]) + std::norm(U[3]) + std::norm(U[6]);
+  n = 1.0 / sqrt(n);
+  U[0] *= n;
+  U[3] *= n;
+  U[6] *= n;
+  const cd p = std::conj(U[0]) * U[1] + std::conj(U[3]) * U[4] + std::conj(U[6]) * U[7];
+  U[1] -= p * U[0];
+  U[4] -= p * U[3];
+  U[7] -= p * U[6];
+  double m = std::norm(U[1]) + std::norm(U[4]) + std::norm(U[7]);
+  m = 1.0 / sqrt(m);
+  U[1] *= m;
+  U[4] *= m;
+  U[7] *= m;
+  U[2] = std::conj(U[3] * U[7] - U[6] * U[4]);
+  U[5] = std::conj(U[6] * U[1] - U[0] * U[7]);
+  U[8] = std::conj(U[0] * U[4] - U[3] * U[1]);
+}
+
+// b <- a b c^dagger (columns 0,1 then completion), Gfix/gtrans.c:32-141
+void gtrans_host(const cd *a, cd *b, const cd *c) {
+  cd t[9], o[9];
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 2; s++)
+      t[r * 3 + s] = b[r * 3] * std::conj(c[s * 3]) + b[r * 3 + 1] * std::conj(c[s * 3 + 1]) +
+                     b[r * 3 + 2] * std::conj(c[s * 3 + 2]);
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 2; s++) o[r * 3 + s] = a[r * 3] * t[s] + a[r * 3 + 1] * t[3 + s] + a[r * 3 + 2] * t[6 + s];
+  o[2] = std::conj(o[3] * o[7] - o[4] * o[6]);
+  o[5] = std::conj(o[1] * o[6] - o[0] * o[7]);
+  o[8] = std::conj(o[0] * o[4] - o[1] * o[3]);
+  memcpy(b, o, sizeof(o));
+}
+
+void random_transform(std::vector<double> &h) {
+  rng_init();
+  const size_t V = volume();
+  const int nth = (int)g_rng.x.size();
+  std::vector<cd> G(V * 9);
+  for (int t = 0; t < nth; t++) {  // libgomp static blocks
+    size_t q = V / nth, r = V % nth;
+    if ((size_t)t < r) {
+      q++;
+      r = 0;
+    }
+    const size_t lo = q * t + r, hi = lo + q;
+    for (size_t i = lo; i < hi; i++) {
+      for (int k = 0; k < 9; k++) G[i * 9 + k] = polar_box(t);
+      reunit_host(&G[i * 9]);
+    }
+  }
+  size_t S[4] = {1, Latt.dims[0], Latt.dims[0] * Latt.dims[1], Latt.dims[0] * Latt.dims[1] * Latt.dims[2]};
+  for (size_t i = 0; i < V; i++) {
+    size_t x[4], r = i;
+    for (int mu = 0; mu < 4; mu++) {
+      x[mu] = r % Latt.dims[mu];
+      r /= Latt.dims[mu];
+    }
+    for (int mu = 0; mu < 4; mu++) {
+      const size_t f = x[mu] + 1 < Latt.dims[mu] ? i + S[mu] : i - (Latt.dims[mu] - 1) * S[mu];
+      gtrans_host(&G[i * 9], (cd *)&h[(i * 4 + mu) * 18], &G[f * 9]);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// NERSC reader for the restarts (IO/read_headers.c:229-362,
+// IO/readers.c:51-58, 170-302): 3x2 or 3x3, IEEE32/64, either endianness
+// ------------------------------------------------------------------------
+int read_nersc(const char *path, std::vector<double> &h) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  char line[1024];
+  std::string hdr;
+  bool end = false;
+  while (fgets(line, sizeof(line), f)) {
+    hdr += line;
+    if (!strncmp(line, "END_HEADER", 10)) {
+      end = true;
+      break;
+    }
+  }
+  auto value = [&](const char *key) -> std::string {
+    size_t p = 0;
+    const std::string k(key);
+    while ((p = hdr.find(k, p)) != std::string::npos) {
+      if (p == 0 || hdr[p - 1] == '\n') {
+        size_t q = p + k.size();
+        while (q < hdr.size() && (hdr[q] == ' ' || hdr[q] == '\t')) q++;
+        if (q < hdr.size() && hdr[q] == '=') {
+          q++;
+          while (q < hdr.size() && (hdr[q] == ' ' || hdr[q] == '\t')) q++;
+          size_t e = hdr.find('\n', q);
+          std::string v = hdr.substr(q, e - q);
+          while (!v.empty() && (v.back() == ' ' || v.back() == '\r' || v.back() == '\t')) v.pop_back();
+          return v;
+        }
+      }
+      p += k.size();
+    }
+    return "";
+  };
+  if (!end) {
+    fclose(f);
+    return -2;
+  }
+  const std::string fp = value("FLOATING_POINT"), dt = value("DATATYPE");
+  const bool dbl = fp.find("64") != std::string::npos;
+  const bool big = fp.find("BIG") != std::string::npos || fp == "IEEE32";
+  const bool two = dt == "4D_SU3_GAUGE";
+  if (!two && dt != "4D_SU3_GAUGE_3x3") {
+    fclose(f);
+    return -3;
+  }
+  const size_t V = volume(), per = two ? 12 : 18, n = V * 4 * per, w = dbl ? 8 : 4;
+  std::vector<unsigned char> buf(n * w);
+  if (fread(buf.data(), w, n, f) != n) {
+    fclose(f);
+    return -4;
+  }
+  fclose(f);
+  h.assign(V * 72, 0.0);
+  for (size_t k = 0; k < n; k++) {
+    double d;
+    if (dbl) {
+      uint64_t u;
+      memcpy(&u, &buf[8 * k], 8);
+      if (big) u = __builtin_bswap64(u);
+      memcpy(&d, &u, 8);
+    } else {
+      uint32_t u;
+      float fl;
+      memcpy(&u, &buf[4 * k], 4);
+      if (big) u = __builtin_bswap32(u);
+      memcpy(&fl, &u, 4);
+      d = fl;
+    }
+    h[(k / per) * 18 + k % per] = d;
+  }
+  if (two)
+    for (size_t l = 0; l < V * 4; l++) {
+      cd *O = (cd *)&h[l * 18];
+      O[6] = std::conj(O[1] * O[5] - O[2] * O[4]);
+      O[7] = std::conj(O[2] * O[3] - O[0] * O[5]);
+      O[8] = std::conj(O[0] * O[4] - O[1] * O[3]);
+    }
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// report helpers
+// ------------------------------------------------------------------------
+// -i log(U) of a unitary 3x3 matrix from its eigen-decomposition (the
+// "Log" column of the temporal-constancy line; GLU uses exact_log_slow)
+void log_herm(const cd *U, cd *Q) {
+  // characteristic polynomial l^3 - a l^2 + b l - d
+  const cd a = U[0] + U[4] + U[8];
+  const cd b = U[0] * U[4] - U[1] * U[3] + U[0] * U[8] - U[2] * U[6] + U[4] * U[8] - U[5] * U[7];
+  const cd d = U[0] * (U[4] * U[8] - U[5] * U[7]) - U[1] * (U[3] * U[8] - U[5] * U[6]) +
+               U[2] * (U[3] * U[7] - U[4] * U[6]);
+  cd l[3] = {cd(1, 0), cd(-0.5, 0.8660254037844386), cd(-0.5, -0.8660254037844386)};
+  for (int it = 0; it < 200; it++) {  // Durand-Kerner
+    for (int k = 0; k < 3; k++) {
+      cd num = ((l[k] - a) * l[k] + b) * l[k] - d, den = 1.0;
+      for (int j = 0; j < 3; j++)
+        if (j != k) den *= (l[k] - l[j]);
+      if (std::abs(den) > 0) l[k] -= num / den;
+    }
+  }
+  for (int k = 0; k < 9; k++) Q[k] = 0.0;
+  for (int k = 0; k < 3; k++) {
+    // eigenvector: cross product of two rows of (U - l I)
+    cd M[9];
+    for (int e = 0; e < 9; e++) M[e] = U[e];
+    M[0] -= l[k];
+    M[4] -= l[k];
+    M[8] -= l[k];
+    cd best[3] = {0, 0, 0};
+    double bn = -1;
+    for (int r1 = 0; r1 < 3; r1++)
+      for (int r2 = r1 + 1; r2 < 3; r2++) {
+        const cd *p = M + 3 * r1, *q = M + 3 * r2;
+        cd v[3] = {std::conj(p[1] * q[2] - p[2] * q[1]), std::conj(p[2] * q[0] - p[0] * q[2]),
+                   std::conj(p[0] * q[1] - p[1] * q[0])};
+        const double nn = std::norm(v[0]) + std::norm(v[1]) + std::norm(v[2]);
+        if (nn > bn) {
+          bn = nn;
+          best[0] = v[0];
+          best[1] = v[1];
+          best[2] = v[2];
+        }
+      }
+    if (bn <= 0) continue;
+    const double s = 1.0 / sqrt(bn);
+    const double th = std::arg(l[k]);
+    for (int r = 0; r < 3; r++)
+      for (int c2 = 0; c2 < 3; c2++) Q[r * 3 + c2] += th * (best[r] * s) * std::conj(best[c2] * s);
+  }
+}
+
+// Gfix/gftests.c:26-130: average difference of the slice sums of the
+// traceless Hermitian projection of the temporal links from slice 0
+void const_time(const std::vector<double> &h, double *lin, double *lg) {
+  const size_t Lcu = Latt.dims[0] * Latt.dims[1] * Latt.dims[2], T = Latt.dims[3];
+  auto slice_sum = [&](size_t t, cd *s, cd *sl) {
+    for (int k = 0; k < 9; k++) s[k] = sl[k] = 0.0;
+    for (size_t j = Lcu * t; j < Lcu * t + Lcu; j++) {
+      const cd *U = (const cd *)&h[(j * 4 + 3) * 18];
+      cd A[9];
+      const double i0 = U[0].imag(), i4 = U[4].imag(), i8 = U[8].imag();
+      A[0] = (2 * i0 - i4 - i8) / 3.0;
+      A[1] = cd(0, -1) * (U[1] - std::conj(U[3])) / 2.0;
+      A[2] = cd(0, -1) * (U[2] - std::conj(U[6])) / 2.0;
+      A[4] = (2 * i4 - i0 - i8) / 3.0;
+      A[5] = cd(0, -1) * (U[5] - std::conj(U[7])) / 2.0;
+      for (int k : {0, 1, 2, 4, 5}) s[k] += A[k];
+      cd Q[9];
+      log_herm(U, Q);
+      for (int k = 0; k < 9; k++) sl[k] += Q[k];
+    }
+  };
+  cd s1[9], l1[9];
+  slice_sum(0, s1, l1);
+  double av = 0, avl = 0;
+  for (size_t t = 1; t < T; t++) {
+    cd s2[9], l2[9];
+    slice_sum(t, s2, l2);
+    av += fabs(s1[0].real() - s2[0].real()) + std::abs(s1[1] - s2[1]) + std::abs(s1[2] - s2[2]) +
+          fabs(s1[4].real() - s2[4].real()) + std::abs(s1[5] - s2[5]);
+    avl += fabs(l1[0].real() - l2[0].real()) + std::abs(l1[1] - l2[1]) + std::abs(l1[2] - l2[2]) +
+           fabs(l1[4].real() - l2[4].real()) + std::abs(l1[5] - l2[5]);
+  }
+  const double check = 1.0 / (9.0 * (double)(T - 1));
+  *lin = av * check;
+  *lg = avl * check;
+}
+
+// Gfix/Landau.c:43-66
+void output_fixing_info(glu_gfix_ctx *c, struct site *lat, double theta, size_t iters) {
+  glu_gfix_reunit(c);
+  double plaq = 0, tr = 0, mx = 0, func = 0, tl = 0;
+  glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
+  std::vector<double> h(volume() * 72);
+  glu_gfix_get_links(c, h.data());
+  unpack(lat, h);
+  printf("[GF] Plaquette :: %1.15f \n[GF] Accuracy :: %1.4e\n", plaq, theta);
+  printf("[GF] Iters :: %zu\n[GF] Link trace :: %1.15f || Maximum :: %1.15f\n", iters, tr, mx);
+  double lin = 0, lg = 0;
+  const_time(h, &lin, &lg);
+  printf("[GF] Temporal constance || Lin %e || Log %e \n", lin, lg);
+  printf("[GF] Functional :: %1.15f\n", func);
+}
+
+size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int method) {
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
+  if (!c) return 123456789;
+  std::vector<double> h;
+  pack(lat, h);
+  size_t iters = 0;
+  if (glu_gfix_set_links(c, h.data()) || glu_gfix_run(c, method, Latt.gf_alpha, acc, max_iters, th, &iters)) {
+    fprintf(stderr, "[GF] device gauge fixing failed: %s\n", glu_gfix_last_error());
+    return 123456789;
+  }
+  glu_gfix_get_links(c, h.data());
+  unpack(lat, h);
+  return iters;
+}
+
+}  // namespace
+
+extern "C" size_t FACG(struct site *lat, struct fftw_stuff *FFTW, double *th, const double acc,
+                       const size_t max_iters) {
+  return fa_call(lat, th, acc, max_iters, GLU_GFIX_CG);
+}
+
+extern "C" size_t FASD(struct site *lat, struct fftw_stuff *FFTW, double *th, const double acc,
+                       const size_t max_iters) {
+  return fa_call(lat, th, acc, max_iters, GLU_GFIX_SD);
+}
+
+// Gfix/Landau.c:261-374 (the FACG callback; FASD with GLU_GFIX_SD)
+extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t max_iter, const char *infile) {
+  const int method = getenv("GLU_GFIX_SD") ? GLU_GFIX_SD : GLU_GFIX_CG;
+  double theta = 0.0;
+  size_t iters = fa_call(lat, &theta, accuracy, max_iter, method);
+  size_t failure = 0;
+  if (iters == 123456789) {
+    fprintf(stderr, "\n[GF] Non-convergence ... Randomly-restarting \n\n[GF] Failure :: %zu || Iters :: %zu || ACC :: %e \n",
+            failure, max_iter, theta);
+    for (failure = 1; failure < 7; failure++) {
+      std::vector<double> h;
+      if (!infile || read_nersc(infile, h) != 0) {
+        printf("[IO] Something funky happened when trying to read in config again!\n");
+        break;
+      }
+      random_transform(h);
+      unpack(lat, h);
+      const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method);
+      if (loc != 123456789) {
+        iters += loc;
+        break;
+      }
+      iters += max_iter;
+      fprintf(stderr, "\n[GF] Non-convergence ... Randomly-restarting \n\n[GF] Failure :: %zu || Iters :: %zu || ACC :: %e \n",
+              failure, loc, theta);
+    }
+  }
+  if (failure == 7) {
+    printf("\n[GF] Failure to converge to a sufficient solution \n[GF] Please lower the GF_TUNE parameter from %g \n",
+           Latt.gf_alpha);
+    return (size_t)-1;
+  }
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
+  if (c) {
+    std::vector<double> h;
+    pack(lat, h);
+    glu_gfix_set_links(c, h.data());
+    output_fixing_info(c, lat, theta, iters);
+  }
+  return iters;
+}
+
+// Gfix/CFACG.c:532-636 with the slices batched on the device
+extern "C" size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW, const double accuracy, const size_t max_iter,
+                             const GLU_bool facg) {
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_COULOMB);
+  if (!c) return 0;
+  std::vector<double> h;
+  pack(lat, h);
+  size_t iters = 0;
+  double th = 0.0;
+  if (glu_gfix_set_links(c, h.data()) ||
+      glu_gfix_run(c, facg == GLU_TRUE ? GLU_GFIX_CG : GLU_GFIX_SD, Latt.gf_alpha, accuracy, max_iter, &th, &iters)) {
+    fprintf(stderr, "[GF] device Coulomb gauge fixing failed: %s\n", glu_gfix_last_error());
+    return 0;
+  }
+  glu_gfix_get_links(c, h.data());
+  unpack(lat, h);
+  return iters;
+}
+
+// Gfix/Coulomb.c:33-100
+extern "C" size_t Coulomb(struct site *lat, const double accuracy, const size_t iter) {
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_COULOMB);
+  if (!c) return 0;
+  std::vector<double> h;
+  pack(lat, h);
+  glu_gfix_set_links(c, h.data());
+  double plaq = 0, tr = 0, mx = 0, func = 0, tl = 0;
+  glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
+  double sl = (4 * tr - tl) / 3.0;
+  fprintf(stdout, "[GF] Initial Tlink :: %1.15f || Slink :: %1.15f \n[GF] Plaquette :: %1.15f \n", tl, sl, plaq);
+  const size_t iters =
+      Coulomb_FA(lat, nullptr, accuracy, iter, getenv("GLU_GFIX_SD") ? GLU_FALSE : GLU_TRUE);
+  pack(lat, h);
+  glu_gfix_set_links(c, h.data());
+  glu_gfix_reunit(c);
+  glu_gfix_get_links(c, h.data());
+  unpack(lat, h);
+  glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
+  sl = (4 * tr - tl) / 3.0;
+  fprintf(stdout,
+          "[GF] Tuning :: %f || Iterations :: %zu ||\n[GF] Final Tlink :: %1.15f || Slink :: %1.15f \n[GF] Plaquette :: "
+          "%1.15f \n",
+          Latt.gf_alpha, iters, tl, sl, plaq);
+  return iters;
+}

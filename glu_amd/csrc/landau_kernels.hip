@@ -1,0 +1,589 @@
+// landau_kernels.hip -- site-parallel streaming kernels of the Landau
+// FACG/FASD iteration: derivative + theta, Polak-Ribiere sums, CG update,
+// three-probe line search, gauge update, control kernels, and the
+// link (re)packing / measurement kernels around them.
+//
+// One thread per lattice site, 256-thread blocks, SoA planes (coalesced),
+// FP64 per-block partial sums reduced by a single-block kernel in a fixed
+// order (bitwise reproducible, independent of what else runs on the GPU).
+#include "gfix_kernels.h"
+
+namespace gfix {
+
+// ---------------------------------------------------------------------------
+// derivative: Gfix/lin_derivs.c:75-193 + packing Gfix/FACG.c:38-73
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_deriv(const vec2<T> *__restrict__ U, vec2<T> *__restrict__ F,
+                                                 double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
+  if (ctrl && ctrl->done) return;
+  const long long V = g.V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double loc[1] = {0.0};
+  if (i < V) {
+    int x[4];
+    coords(g, i, x);
+    T R0 = 0, R1 = 0, I1 = 0, R2 = 0, I2 = 0, R3 = 0, R4 = 0, I4 = 0;
+#pragma unroll
+    for (int mu = 0; mu < kND; mu++) {
+      const su3<T> q = load_link<T>(U, mu, i, V);
+      const su3<T> s = load_link<T>(U, mu, nb_bwd(g, i, x, mu), V);
+      R0 += T(2) * s.e[0].im - s.e[4].im - s.e[8].im;
+      R0 += T(-2) * q.e[0].im + q.e[4].im + q.e[8].im;
+      R1 += s.e[1].re - s.e[3].re - q.e[1].re + q.e[3].re;
+      I1 += s.e[1].im + s.e[3].im - q.e[1].im - q.e[3].im;
+      R2 += s.e[2].re - s.e[6].re - q.e[2].re + q.e[6].re;
+      I2 += s.e[2].im + s.e[6].im - q.e[2].im - q.e[6].im;
+      R3 += T(2) * s.e[4].im - s.e[0].im - s.e[8].im;
+      R3 += T(-2) * q.e[4].im + q.e[0].im + q.e[8].im;
+      R4 += s.e[5].re - s.e[7].re - q.e[5].re + q.e[7].re;
+      I4 += s.e[5].im + s.e[7].im - q.e[5].im - q.e[7].im;
+    }
+    st<T>(F + i, mk<T>(-R3 / T(3), R0 / T(3)));
+    st<T>(F + V + i, mk<T>(T(0.5) * R1, T(0.5) * I1));
+    st<T>(F + 2 * V + i, mk<T>(T(0.5) * R2, T(0.5) * I2));
+    st<T>(F + 3 * V + i, mk<T>(T(0.5) * R4, T(0.5) * I4));
+    const double r0 = R0, r3 = R3;
+    loc[0] = 0.4444444444444444 * (r0 * r0 + r3 * (r3 + r0)) +
+             ((double)R1 * R1 + (double)I1 * I1 + (double)R2 * R2 + (double)I2 * I2 + (double)R4 * R4 +
+              (double)I4 * I4);
+  }
+  block_sum_store<1>(loc, part + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// trace-metric sums of the packed field: Gfix/FACG.c:99-178
+//   s1 = <in, in>,  s2 = <in, in - old>
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pr_sums(const vec2<T> *__restrict__ F, const vec2<T> *__restrict__ O,
+                                                   double *__restrict__ part, long long V,
+                                                   const Ctrl *__restrict__ ctrl) {
+  if (ctrl && ctrl->done) return;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {0.0, 0.0};
+  if (i < V) {
+    double s1 = 0.0, s2 = 0.0;
+    const cx<T> z0t = ld<T>(F + i), o0t = ld<T>(O + i);
+    const double zr = z0t.re, zi = z0t.im, tr = zr - (double)o0t.re, ti = zi - (double)o0t.im;
+    s1 += 2.0 * (zr * zr + zi * zi + zr * zi);
+    s2 += 2.0 * (zr * tr + zi * ti);
+    s2 += zr * ti + zi * tr;
+#pragma unroll
+    for (int f = 1; f < 4; f++) {
+      const cx<T> z = ld<T>(F + f * V + i), o = ld<T>(O + f * V + i);
+      const double ar = z.re, ai = z.im;
+      s1 += 2.0 * (ar * ar + ai * ai);
+      s2 += 2.0 * (ar * (ar - (double)o.re) + ai * (ai - (double)o.im));
+    }
+    v[0] = s1;
+    v[1] = s2;
+  }
+  block_sum_store<2>(v, part + 2 * blockIdx.x);
+}
+
+// sn = in + beta sn ; old = in   (Gfix/FACG.c:315-323)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_update(const vec2<T> *__restrict__ F, vec2<T> *__restrict__ sn,
+                                                     vec2<T> *__restrict__ O, long long n4,
+                                                     const Ctrl *__restrict__ ctrl) {
+  if (ctrl->done) return;
+  const T beta = (T)ctrl->beta;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += (long long)gridDim.x * blockDim.x) {
+    const cx<T> a = ld<T>(F + j), b = ld<T>(sn + j);
+    st<T>(sn + j, mk<T>(a.re + beta * b.re, a.im + beta * b.im));
+    st<T>(O + j, a);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_copy3(const vec2<T> *__restrict__ F, vec2<T> *__restrict__ a,
+                                                 vec2<T> *__restrict__ b, long long n4,
+                                                 const Ctrl *__restrict__ ctrl) {
+  if (ctrl->done) return;
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += (long long)gridDim.x * blockDim.x) {
+    const vec2<T> v = F[j];
+    a[j] = v;
+    b[j] = v;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ su3<T> gauge_at(const vec2<T> *__restrict__ F, long long V, long long i, T alpha) {
+  return gauge_matrix<T>(ld<T>(F + i), ld<T>(F + V + i), ld<T>(F + 2 * V + i), ld<T>(F + 3 * V + i), alpha);
+}
+
+// ---------------------------------------------------------------------------
+// line search probes: Gfix/line_search.c:274-341 (probes PL1 = 0.15,
+// PL2 = 0.30 at :34-35).  Skipped (no partials) when theta <= CG_TOL unless
+// forced (the SD start always searches).  The probes are evaluations of the
+// functional, so they are computed in FP64 whatever the storage type: in
+// FP32 the rounding of the reunitarised probe matrices biases the O(12 V)
+// sums by more than the differences the parabola fit needs.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ su3<double> promote(const su3<T> &m) {
+  su3<double> d;
+#pragma unroll
+  for (int k = 0; k < 9; k++) d.e[k] = mk<double>((double)m.e[k].re, (double)m.e[k].im);
+  return d;
+}
+template <typename T>
+__device__ __forceinline__ su3<double> gauge_at64(const vec2<T> *__restrict__ F, long long V, long long i,
+                                                  double alpha) {
+  auto f = [&](long long k) {
+    const cx<T> z = ld<T>(F + k);
+    return mk<double>((double)z.re, (double)z.im);
+  };
+  return gauge_matrix<double>(f(i), f(V + i), f(2 * V + i), f(3 * V + i), alpha);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restrict__ U, const vec2<T> *__restrict__ F,
+                                                       double *__restrict__ part, Geo g,
+                                                       const Ctrl *__restrict__ ctrl, int force) {
+  if (ctrl->done) return;
+  if (!force && !(ctrl->theta > ctrl->cg_tol)) return;
+  const long long V = g.V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (i < V) {
+    int x[4];
+    coords(g, i, x);
+    const su3<double> A1 = gauge_at64<T>(F, V, i, 0.15);
+    const su3<double> A2 = gauge_at64<T>(F, V, i, 0.30);
+#pragma unroll
+    for (int mu = 0; mu < kND; mu++) {
+      const su3<double> u = promote<T>(load_link<T>(U, mu, i, V));
+      const long long f = nb_fwd(g, i, x, mu);
+      v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
+      const su3<double> B1 = gauge_at64<T>(F, V, f, 0.15);
+      v[1] += retrace_abc_dag<double>(A1, u, B1);
+      const su3<double> B2 = gauge_at64<T>(F, V, f, 0.30);
+      v[2] += retrace_abc_dag<double>(A2, u, B2);
+    }
+  }
+  block_sum_store<3>(v, part + 3 * blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// U_mu(x) <- g(x) U_mu(x) g(x+mu)^dagger: Gfix/line_search.c:60-97.
+// With G != null the site transform is also accumulated in FP64,
+// G(x) <- g(x) G(x) with g rebuilt in FP64 from the same field values: the
+// mixed schedule keeps the FP64 links untouched during the FP32 phase and
+// replays G on them exactly at the switch.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, const vec2<T> *__restrict__ F,
+                                                  double2 *__restrict__ G, Geo g, const Ctrl *__restrict__ ctrl,
+                                                  int fixed_step) {
+  if (ctrl->done) return;
+  const long long V = g.V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  const double alpha = fixed_step ? ctrl->gf_alpha : ctrl->alpha;
+  int x[4];
+  coords(g, i, x);
+  const su3<T> A = gauge_at<T>(F, V, i, (T)alpha);
+#pragma unroll
+  for (int mu = 0; mu < kND; mu++) {
+    const su3<T> B = gauge_at<T>(F, V, nb_fwd(g, i, x, mu), (T)alpha);
+    const su3<T> u = load_link<T>(U, mu, i, V);
+    store_link<T>(U, mu, i, V, gtransform<T>(A, u, B));
+  }
+  if (G) {
+    const su3<double> A64 = gauge_at64<T>(F, V, i, alpha);
+    store_link<double>(G, 0, i, V, mul_cols<double>(A64, load_link<double>(G, 0, i, V)));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// single-block finalisers (fixed summation order)
+// ---------------------------------------------------------------------------
+template <int NV>
+__device__ __forceinline__ void final_sum(const double *__restrict__ part, long long nb, double (&out)[NV]) {
+  __shared__ double sh[NV][1024 / kWave];
+  double v[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = 0.0;
+  for (long long b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] += part[b * NV + k];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) sh[k][w] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = 0.0;
+    for (int j = 0; j < (int)(blockDim.x / kWave); j++) s += sh[k][j];
+    out[k] = s;
+  }
+}
+
+// theta = sum / (NC V): GFNORM_LANDAU, Headers/GLU_definitions.h:427
+__global__ void k_fin_theta(const double *__restrict__ part, long long nb, Ctrl *ctrl, double norm) {
+  if (ctrl->done) return;
+  double s[1];
+  final_sum<1>(part, nb, s);
+  if (threadIdx.x == 0) {
+    double th = s[0] * norm;
+    if (ctrl->sd_start) th += ctrl->stale_theta;  // GLU's stale red[5] slots
+    ctrl->theta = th;
+  }
+}
+
+// beta: Polak-Ribiere clipped at 0, Fletcher-Reeves below CG_TOL
+// (Gfix/FACG.c:306-313)
+__global__ void k_fin_pr(const double *__restrict__ part, long long nb, Ctrl *ctrl) {
+  if (ctrl->done) return;
+  double s[2];
+  final_sum<2>(part, nb, s);
+  if (threadIdx.x == 0) {
+    const double insum = s[0], sc = s[1];
+    double beta = sc / ctrl->inold;
+    if (beta < 0.0) beta = 0.0;
+    if (ctrl->theta < ctrl->cg_tol) beta = insum / ctrl->inold;
+    ctrl->inold = insum;
+    ctrl->insum = insum;
+    ctrl->sum_conj = sc;
+    ctrl->beta = beta;
+  }
+}
+
+// end of the SD start: inold = sum_DER3 (+ GLU's stale red[3] slots)
+__global__ void k_fin_sd(const double *__restrict__ part, long long nb, Ctrl *ctrl) {
+  if (ctrl->done) return;
+  double s[2];
+  final_sum<2>(part, nb, s);
+  if (threadIdx.x == 0) {
+    ctrl->inold = s[0] + ctrl->stale_insum;
+    ctrl->beta = 0.0;
+    if (ctrl->trace_n < ctrl->trace_cap) ctrl->trace_n++;  // row already written
+    ctrl->sd_start = 0;
+    ctrl->stale_insum = 0.0;
+    ctrl->stale_theta = 0.0;
+    ctrl->stale_probe[0] = ctrl->stale_probe[1] = ctrl->stale_probe[2] = 0.0;
+  }
+}
+
+// parabola vertex through the three probes, clamped: Gfix/line_search.c:38-57
+__device__ __forceinline__ double approx_minimum(const double al[3], const double f[3]) {
+  const double A = f[0] / ((al[0] - al[1]) * (al[0] - al[2]));
+  const double B = f[1] / ((al[1] - al[0]) * (al[1] - al[2]));
+  const double C = f[2] / ((al[2] - al[0]) * (al[2] - al[1]));
+  const double sol = (al[0] * (B + C) + al[1] * (A + C) + al[2] * (A + B)) / (2 * (A + B + C));
+  if (sol > al[2]) return al[2];
+  if (isnan(sol) || isinf(sol) || sol < al[0]) return 0.08;
+  return sol;
+}
+
+__global__ void k_fin_ls(const double *__restrict__ part, long long nb, Ctrl *ctrl, double *trace, int force,
+                         double pl1, double pl2) {
+  if (ctrl->done) return;
+  const int do_ls = force || (ctrl->theta > ctrl->cg_tol);
+  double s[3] = {0.0, 0.0, 0.0};
+  if (do_ls) final_sum<3>(part, nb, s);
+  if (threadIdx.x == 0) {
+    if (do_ls) {
+      double val[3];
+      for (int k = 0; k < 3; k++) {
+        const double tot = s[k] + (ctrl->sd_start ? ctrl->stale_probe[k] : 0.0);
+        ctrl->probe[k] = tot;
+        val[k] = -tot;
+      }
+      const double al[3] = {0.0, pl1, pl2};
+      ctrl->alpha = approx_minimum(al, val);
+      ctrl->did_ls = 1;
+    } else {
+      ctrl->alpha = ctrl->gf_alpha;
+      ctrl->did_ls = 0;
+    }
+    if (ctrl->sd_start && trace && ctrl->trace_n < ctrl->trace_cap) {
+      double *r = trace + 3 * ctrl->trace_n;
+      r[0] = ctrl->theta;
+      r[1] = ctrl->alpha;
+      r[2] = 0.0;
+    }
+  }
+}
+
+// end of a loop iteration: count, record, stop test (Gfix/FACG.c:331-334)
+__global__ void k_iter_end(Ctrl *ctrl, double *trace, int fasd) {
+  if (ctrl->done) return;
+  if (fasd) {
+    ctrl->alpha = ctrl->gf_alpha;
+    ctrl->beta = 0.0;
+  }
+  ctrl->iters++;
+  ctrl->total++;
+  if (trace && ctrl->trace_n < ctrl->trace_cap) {
+    double *r = trace + 3 * ctrl->trace_n;
+    r[0] = ctrl->theta;
+    r[1] = ctrl->alpha;
+    r[2] = ctrl->beta;
+    ctrl->trace_n++;
+  }
+  ctrl->stale_insum = ctrl->insum;
+  ctrl->stale_theta = ctrl->theta;
+  for (int k = 0; k < 3; k++) ctrl->stale_probe[k] = ctrl->did_ls ? ctrl->probe[k] : 0.0;
+  if (!(ctrl->theta > ctrl->acc && ctrl->iters < ctrl->max_iters)) ctrl->done = 1;
+  if (ctrl->switch_theta > 0.0 && ctrl->theta < ctrl->switch_theta) {
+    ctrl->switched = 1;
+    ctrl->done = 1;
+  }
+}
+
+__global__ void k_begin_segment(Ctrl *ctrl, int sd) {
+  ctrl->done = 0;
+  ctrl->iters = 0;
+  ctrl->sd_start = sd;
+  ctrl->did_ls = 0;
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion: GLU AoS double[V][4][18] <-> device column planes
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pack(const double *__restrict__ aos, vec2<T> *__restrict__ U,
+                                                long long V, int nd, long long site0, long long nsite) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsite) return;
+  const long long i = site0 + j;
+  for (int mu = 0; mu < nd; mu++) {
+    const double *a = aos + (j * nd + mu) * 18;
+    su3<T> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k] = mk<T>((T)a[2 * k], (T)a[2 * k + 1]);
+    store_link<T>(U, mu, i, V, m);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_unpack(const vec2<T> *__restrict__ U, double *__restrict__ aos,
+                                                  long long V, int nd, long long site0, long long nsite) {
+  const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nsite) return;
+  const long long i = site0 + j;
+  for (int mu = 0; mu < nd; mu++) {
+    const su3<T> m = load_link<T>(U, mu, i, V);
+    double *a = aos + (j * nd + mu) * 18;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      a[2 * k] = (double)m.e[k].re;
+      a[2 * k + 1] = (double)m.e[k].im;
+    }
+  }
+}
+
+// FP64 links -> FP32 copy
+__global__ __launch_bounds__(kBlock) void k_demote(const double2 *__restrict__ U, float2 *__restrict__ W, long long n) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    const double2 v = U[j];
+    W[j] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+// identity site transform (column planes of one "direction")
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_identity(vec2<T> *__restrict__ G, long long V) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  su3<T> m;
+#pragma unroll
+  for (int k = 0; k < 9; k++) m.e[k] = mk<T>(T(0), T(0));
+  m.e[0].re = m.e[4].re = m.e[8].re = T(1);
+  store_link<T>(G, 0, i, V, m);
+}
+
+// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dagger in FP64 with the accumulated site
+// transform reunitarised first (Gfix/gtrans.c:169-189)
+__global__ __launch_bounds__(kBlock) void k_apply_gauge(double2 *__restrict__ U, const double2 *__restrict__ G,
+                                                       Geo g) {
+  const long long V = g.V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  int x[4];
+  coords(g, i, x);
+  auto fetch = [&](long long s) {
+    su3<double> d = load_link<double>(G, 0, s, V);
+    gram_reunit<double>(d);
+    return d;
+  };
+  const su3<double> A = fetch(i);
+#pragma unroll
+  for (int mu = 0; mu < kND; mu++) {
+    const su3<double> B = fetch(nb_fwd(g, i, x, mu));
+    const su3<double> u = load_link<double>(U, mu, i, V);
+    store_link<double>(U, mu, i, V, gtransform<double>(A, u, B));
+  }
+}
+
+// packed fields FP32 -> FP64 (the CG state crosses the precision switch)
+__global__ __launch_bounds__(kBlock) void k_promote(const float2 *__restrict__ a, double2 *__restrict__ b, long long n) {
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long long)gridDim.x * blockDim.x) {
+    const float2 v = a[j];
+    b[j] = make_double2(v.x, v.y);
+  }
+}
+
+// reunitarise every link (Geometry/random_config.c:43-56)
+__global__ __launch_bounds__(kBlock) void k_reunit(double2 *__restrict__ U, long long V, int nd) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= V) return;
+  for (int mu = 0; mu < nd; mu++) {
+    su3<double> m = load_link<double>(U, mu, i, V);
+    gram_reunit<double>(m);
+    store_link<double>(U, mu, i, V, m);
+  }
+}
+
+// plaquette / link-trace partial sums: Field/plaqs_links.c:243-265, 399-448
+__global__ __launch_bounds__(kBlock) void k_measure(const double2 *__restrict__ U, double *__restrict__ part, Geo g) {
+  const long long V = g.V;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[3] = {0.0, 0.0, 0.0};  // plaquette sum, link trace, temporal trace
+  double mx = -1e300;             // largest single-link trace
+  if (i < V) {
+    int x[4];
+    coords(g, i, x);
+    su3<double> u[4];
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) u[mu] = load_link<double>(U, mu, i, V);
+    for (int mu = 0; mu < 4; mu++) {
+      const double r = retrace<double>(u[mu]);
+      v[1] += r;
+      if (mu == 3) v[2] += r;
+      mx = r > mx ? r : mx;
+      const long long t = nb_fwd(g, i, x, mu);
+      for (int nu = 0; nu < mu; nu++) {
+        const long long s = nb_fwd(g, i, x, nu);
+        const su3<double> b = load_link<double>(U, nu, t, V);
+        const su3<double> c = load_link<double>(U, mu, s, V);
+        // Re tr(U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag)
+        const su3<double> ab = mul_su3<double>(u[mu], b);
+        const su3<double> dc = mul_su3<double>(u[nu], c);
+        double t2 = 0.0;
+        for (int k = 0; k < 9; k++) t2 += ab.e[k].re * dc.e[k].re + ab.e[k].im * dc.e[k].im;
+        v[0] += t2;
+      }
+    }
+  }
+  __shared__ double mxs[kBlock];
+  mxs[threadIdx.x] = mx;
+  block_sum_store<3>(v, part + 4 * blockIdx.x);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = -1e300;
+    for (int k = 0; k < (int)blockDim.x; k++) m = mxs[k] > m ? mxs[k] : m;
+    part[4 * blockIdx.x + 3] = m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations + launch helpers
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+template <typename T>
+void launch_deriv(const void *U, void *F, double *part, const Geo &g, const Ctrl *ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_deriv<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)U, (vec2<T> *)F, part, g,
+                     ctrl);
+}
+template <typename T>
+void launch_pr_sums(const void *F, const void *O, double *part, long long V, const Ctrl *ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_pr_sums<T>, dim3(nblk(V)), dim3(kBlock), 0, s, (const vec2<T> *)F, (const vec2<T> *)O, part,
+                     V, ctrl);
+}
+template <typename T>
+void launch_cg_update(const void *F, void *sn, void *O, long long V, const Ctrl *ctrl, hipStream_t s) {
+  const long long n4 = 4 * V;
+  const unsigned b = (unsigned)std::min<long long>(nblk(n4), 8192);
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)sn, (vec2<T> *)O,
+                     n4, ctrl);
+}
+template <typename T>
+void launch_copy3(const void *F, void *a, void *b2, long long V, const Ctrl *ctrl, hipStream_t s) {
+  const long long n4 = 4 * V;
+  const unsigned b = (unsigned)std::min<long long>(nblk(n4), 8192);
+  hipLaunchKernelGGL(k_copy3<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)a, (vec2<T> *)b2, n4,
+                     ctrl);
+}
+template <typename T>
+void launch_line_search(const void *U, const void *F, double *part, const Geo &g, const Ctrl *ctrl, int force,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_line_search<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)U, (const vec2<T> *)F,
+                     part, g, ctrl, force);
+}
+template <typename T>
+void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s) {
+  hipLaunchKernelGGL(k_egauge<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (vec2<T> *)U, (const vec2<T> *)F,
+                     (double2 *)G, g, ctrl, fixed);
+}
+template <typename T>
+void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack<T>, dim3(nblk(n)), dim3(kBlock), 0, s, aos, (vec2<T> *)U, V, nd, site0, n);
+}
+template <typename T>
+void launch_unpack(const void *U, double *aos, long long V, int nd, long long site0, long long n, hipStream_t s) {
+  hipLaunchKernelGGL(k_unpack<T>, dim3(nblk(n)), dim3(kBlock), 0, s, (const vec2<T> *)U, aos, V, nd, site0, n);
+}
+template <typename T> void launch_identity(void *G, long long V, hipStream_t s) {
+  hipLaunchKernelGGL(k_identity<T>, dim3(nblk(V)), dim3(kBlock), 0, s, (vec2<T> *)G, V);
+}
+
+#define GFIX_INST(T)                                                                                          \
+  template void launch_deriv<T>(const void *, void *, double *, const Geo &, const Ctrl *, hipStream_t);      \
+  template void launch_pr_sums<T>(const void *, const void *, double *, long long, const Ctrl *, hipStream_t); \
+  template void launch_cg_update<T>(const void *, void *, void *, long long, const Ctrl *, hipStream_t);       \
+  template void launch_copy3<T>(const void *, void *, void *, long long, const Ctrl *, hipStream_t);           \
+  template void launch_line_search<T>(const void *, const void *, double *, const Geo &, const Ctrl *, int,   \
+                                      hipStream_t);                                                           \
+  template void launch_egauge<T>(void *, const void *, void *, const Geo &, const Ctrl *, int, hipStream_t);  \
+  template void launch_pack<T>(const double *, void *, long long, int, long long, long long, hipStream_t);     \
+  template void launch_unpack<T>(const void *, double *, long long, int, long long, long long, hipStream_t);   \
+  template void launch_identity<T>(void *, long long, hipStream_t);
+GFIX_INST(double)
+GFIX_INST(float)
+
+void launch_fin_theta(const double *part, long long nb, Ctrl *ctrl, double norm, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_theta, dim3(1), dim3(1024), 0, s, part, nb, ctrl, norm);
+}
+void launch_fin_pr(const double *part, long long nb, Ctrl *ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_pr, dim3(1), dim3(1024), 0, s, part, nb, ctrl);
+}
+void launch_fin_sd(const double *part, long long nb, Ctrl *ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_sd, dim3(1), dim3(1024), 0, s, part, nb, ctrl);
+}
+void launch_fin_ls(const double *part, long long nb, Ctrl *ctrl, double *trace, int force, double pl1, double pl2,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_ls, dim3(1), dim3(1024), 0, s, part, nb, ctrl, trace, force, pl1, pl2);
+}
+void launch_iter_end(Ctrl *ctrl, double *trace, int fasd, hipStream_t s) {
+  hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctrl, trace, fasd);
+}
+void launch_begin_segment(Ctrl *ctrl, int sd, hipStream_t s) {
+  hipLaunchKernelGGL(k_begin_segment, dim3(1), dim3(1), 0, s, ctrl, sd);
+}
+void launch_demote(const void *U, void *W, long long n, hipStream_t s) {
+  const unsigned b = (unsigned)std::min<long long>(nblk(n), 16384);
+  hipLaunchKernelGGL(k_demote, dim3(b), dim3(kBlock), 0, s, (const double2 *)U, (float2 *)W, n);
+}
+void launch_apply_gauge(void *U, const void *G, const Geo &g, hipStream_t s) {
+  hipLaunchKernelGGL(k_apply_gauge, dim3(nblk(g.V)), dim3(kBlock), 0, s, (double2 *)U, (const double2 *)G, g);
+}
+void launch_promote(const void *a, void *b, long long n, hipStream_t s) {
+  const unsigned bl = (unsigned)std::min<long long>(nblk(n), 16384);
+  hipLaunchKernelGGL(k_promote, dim3(bl), dim3(kBlock), 0, s, (const float2 *)a, (double2 *)b, n);
+}
+void launch_reunit(void *U, long long V, int nd, hipStream_t s) {
+  hipLaunchKernelGGL(k_reunit, dim3(nblk(V)), dim3(kBlock), 0, s, (double2 *)U, V, nd);
+}
+void launch_measure(const void *U, double *part, const Geo &g, hipStream_t s) {
+  hipLaunchKernelGGL(k_measure, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const double2 *)U, part, g);
+}
+
+}  // namespace gfix

@@ -1,0 +1,195 @@
+/*
+ * glu_gfix.h -- C ABI of the MI355X gauge-fixing library (libglu_gfix.so).
+ *
+ * Two layers:
+ *
+ *  1. GLU drop-in.  Landau(), Coulomb(), FACG(), FASD() and Coulomb_FA()
+ *     with GLU's exact prototypes.  Linked ahead of a libGLU.a built
+ *     without Gfix/{Landau,FACG,CFACG,Coulomb}.o they replace GLU's CPU
+ *     gauge fixing; they read GLU's global `Latt` (dims, gf_alpha, Seed,
+ *     Nthreads) and overwrite the links of `lat` in place, as GLU does.
+ *
+ *  2. Plain-pointer API (glu_gfix_*): a context owning one lattice's
+ *     device state.  Links cross the boundary in GLU's own memory layout,
+ *     double[V][4][18] (site-major with x fastest, then direction, then the
+ *     row-major 3x3 complex matrix as re/im pairs).
+ *
+ * Every glu_gfix_* call returns 0 on success and a negative code on
+ * failure; glu_gfix_last_error() describes the last failure.  No call
+ * falls back to the CPU: without a usable HIP device they fail.
+ */
+#ifndef GLU_GFIX_H
+#define GLU_GFIX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* GLU types, laid out as GLU's Headers/GLU_types.h and GLU_enum.h     */
+/* (skipped when GLU's own headers were included first)                */
+/* ------------------------------------------------------------------ */
+#ifndef GLU_ENUM_H
+typedef enum { GLU_FALSE, GLU_TRUE } GLU_bool; /* Headers/GLU_enum.h:83-85 */
+#endif
+#ifndef GLU_TYPES_H
+/* Headers/GLU_types.h:269-273; O[mu] points at 9 complex doubles */
+struct site {
+  void **O;
+  size_t neighbor[4];
+  size_t back[4];
+};
+/* Headers/GLU_types.h:229-243 (the enum members are int-sized) */
+struct latt_info {
+  size_t dims[4];
+  size_t Lsq;
+  size_t Lcu;
+  size_t Volume;
+  size_t flow;
+  double gf_alpha;
+  double sm_alpha[3];
+  int head;
+  uint32_t Seed[1];
+  double twiddles[4];
+  uint32_t Nthreads;
+  void *su2_data;
+  int argc;
+};
+struct fftw_stuff; /* unused by this backend; callers may pass NULL */
+#endif
+
+/* Landau gauge fixing with restarts and GLU's [GF] report lines.
+   Replaces Gfix/Landau.c:261 (prototype Headers/Landau.h:51).
+   Returns the iteration count, or (size_t)-1 (GLU_FAILURE) when every
+   restart failed. */
+size_t Landau(struct site *lat, const double accuracy, const size_t max_iter,
+              const char *infile);
+
+/* Coulomb gauge fixing of every time slice plus the report lines.
+   Replaces Gfix/Coulomb.c:33 (prototype Headers/Coulomb.h:40).
+   Returns the summed iteration count over slices. */
+size_t Coulomb(struct site *lat, const double accuracy, const size_t iter);
+
+/* Fourier-accelerated Polak-Ribiere CG.  Replaces Gfix/FACG.c:341
+   (prototype Headers/FACG.h:37).  *th receives the final theta; returns the
+   iterations of the last segment or 123456789 on non-convergence. */
+size_t FACG(struct site *lat, struct fftw_stuff *FFTW, double *th,
+            const double acc, const size_t max_iters);
+
+/* Fourier-accelerated steepest descent with the fixed step GF_TUNE (the
+   GLU_GFIX_SD build).  Replaces Gfix/FACG.c:395 (Headers/FACG.h:54). */
+size_t FASD(struct site *lat, struct fftw_stuff *FFTW, double *th,
+            const double acc, const size_t max_iters);
+
+/* Coulomb FACG (FACG == GLU_TRUE) or FASD per time slice, all slices
+   batched on the device.  Replaces Gfix/CFACG.c:532 (Headers/CFACG.h:37). */
+size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW,
+                  const double accuracy, const size_t max_iter,
+                  const GLU_bool FACG);
+
+/* ------------------------------------------------------------------ */
+/* plain-pointer API                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct glu_gfix_ctx glu_gfix_ctx;
+
+enum { GLU_GFIX_LANDAU = 0, GLU_GFIX_COULOMB = 1 };
+enum { GLU_GFIX_FP64 = 0, GLU_GFIX_MIXED = 1 };
+enum { GLU_GFIX_CG = 0, GLU_GFIX_SD = 1 };
+
+/* Context for one lattice of extent dims[0..3] (x, y, z, t).  Allocates
+   the device state of the whole run (links, 3 packed fields, partials).
+   Mirrors the allocations of Gfix/CGalloc.c:98-129 and the FFTW plans of
+   Geometry/plan_ffts.c:103-168, made once per context. */
+int glu_gfix_create(glu_gfix_ctx **ctx, const size_t dims[4], int gftype,
+                    int precision);
+int glu_gfix_destroy(glu_gfix_ctx *ctx);
+
+/* Upload / download links in GLU's layout double[V][4][18] (host). */
+int glu_gfix_set_links(glu_gfix_ctx *ctx, const double *links);
+int glu_gfix_get_links(glu_gfix_ctx *ctx, double *links);
+/* Same, with the GLU-layout buffer already in device memory. */
+int glu_gfix_set_links_device(glu_gfix_ctx *ctx, const double *links_dev);
+
+/* Keep a device copy of the current links / restore it (device to device). */
+int glu_gfix_snapshot(glu_gfix_ctx *ctx);
+int glu_gfix_restore(glu_gfix_ctx *ctx);
+
+/* Run the gauge fixing on the device-resident links.  method: GLU_GFIX_CG
+   (FACG) or GLU_GFIX_SD (FASD).  Landau: FACG()/FASD() semantics (theta
+   test, continuation, 123456789 on failure) written to *iters.  Coulomb:
+   Coulomb_FA() semantics, total iterations.  *theta: final accuracy
+   (Coulomb: largest slice theta).  Synchronous. */
+int glu_gfix_run(glu_gfix_ctx *ctx, int method, double gf_tune,
+                 double accuracy, size_t max_iters, double *theta,
+                 size_t *iters);
+
+/* Trajectory of the last run: rows (theta_k, alpha_k, beta_k); row 0 is the
+   steepest-descent start.  Landau only.  Recording costs nothing extra. */
+int glu_gfix_trace(glu_gfix_ctx *ctx, double *rows, size_t cap, size_t *n);
+
+/* Observables of the device links: average plaquette
+   (Field/plaqs_links.c:243-265), link trace and its maximum single-link
+   value (:424-448), functional 1 - link trace (Gfix/gftests.c:133-147),
+   temporal link trace (:399-422). */
+int glu_gfix_measure(glu_gfix_ctx *ctx, double *plaquette, double *link_trace,
+                     double *link_max, double *functional, double *tlink);
+
+/* theta_test_lin of the device links (Gfix/gftests.c:215-250). */
+int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
+
+/* Reunitarise every device link (Geometry/random_config.c:43-56). */
+int glu_gfix_reunit(glu_gfix_ctx *ctx);
+
+/* Iterations of the last mixed-precision run spent in FP32 / FP64. */
+int glu_gfix_mixed_split(glu_gfix_ctx *ctx, size_t *fp32_iters,
+                         size_t *fp64_iters);
+
+/* Per-slice results of the last Coulomb run (T entries each). */
+int glu_gfix_slice_info(glu_gfix_ctx *ctx, size_t *iters, double *theta);
+
+/* Stage-level entry points (used by the parity tests and the profiler):
+   the derivative + packing of Gfix/FACG.c:38-73 (fields as 4*V complex =
+   8*V doubles, field-major) with its theta; the Fourier acceleration of
+   Gfix/FACG.c:77-95 applied to a host field; the three line-search probe
+   sums and step of Gfix/line_search.c:274-341 for a host field; and the
+   gauge update of Gfix/line_search.c:60-97 with a given step. */
+int glu_gfix_stage_deriv(glu_gfix_ctx *ctx, double *fields, double *theta);
+int glu_gfix_stage_fa(glu_gfix_ctx *ctx, const double *fields_in,
+                      double *fields_out);
+int glu_gfix_stage_line_search(glu_gfix_ctx *ctx, const double *fields,
+                               double probes[3], double *alpha);
+int glu_gfix_stage_egauge(glu_gfix_ctx *ctx, const double *fields,
+                          double alpha);
+
+/* Input generation (not part of GLU's gauge-fixing path; stands in for
+   GLU's own generators, Update/KPHB.c:124-257 and
+   Geometry/random_config.c:61-95, for lattices too large for the host):
+   quenched Wilson heatbath (Cabibbo-Marinari, Kennedy-Pendleton) with `nor`
+   overrelaxation passes per sweep on the device links, and a seeded
+   Haar-like random gauge transform of them.  Counter-based random numbers:
+   the result depends only on the seed. */
+int glu_gfix_heatbath(glu_gfix_ctx *ctx, double beta, int sweeps, int nor,
+                      uint64_t seed, int cold_start);
+int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
+
+/* Per-kernel timing with HIP events on the context's stream.  Kernel
+   classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
+   update, 4 line search, 5 gauge update; +6 for the FP32 variants.
+   profile_read reports the launches that did work (launches issued past
+   convergence return at once and are left out), their mean and total. */
+int glu_gfix_profile(glu_gfix_ctx *ctx, int enable);
+int glu_gfix_profile_read(glu_gfix_ctx *ctx, int kernel, size_t *launches,
+                          double *mean_ms, double *total_ms);
+
+/* Bytes of device memory a context of these dims needs. */
+size_t glu_gfix_footprint(const size_t dims[4], int gftype, int precision);
+
+const char *glu_gfix_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

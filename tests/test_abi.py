@@ -1,0 +1,58 @@
+"""CPU-side checks of the C ABI: the library builds, loads, exports every
+function include/glu_gfix.h declares, and refuses to run without a HIP
+device instead of falling back to the CPU."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+import glu_amd as gl
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "glu_gfix.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
+    skip = {"if", "sizeof", "defined"}
+    return sorted({n for n in names if n not in skip and (n.startswith("glu_gfix") or n[0].isupper())})
+
+
+def test_header_lists_the_glu_entry_points():
+    names = declared_functions()
+    for n in ("Landau", "Coulomb", "FACG", "FASD", "Coulomb_FA"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(gl.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(declared_functions()) == set(gl.EXPORTS)
+
+
+def test_no_cpu_fallback_without_device():
+    h = C.c_void_p()
+    dims = (C.c_size_t * 4)(4, 4, 4, 4)
+    rc = gl.lib().glu_gfix_create(C.byref(h), dims, 0, 0)
+    if rc == 0:  # a device is present: nothing to check here
+        gl.lib().glu_gfix_destroy(h)
+        pytest.skip("HIP device present")
+    assert rc == -3
+    assert b"no HIP device" in gl.lib().glu_gfix_last_error()
+    with pytest.raises(gl.GfixError):
+        gl.Gfix((4, 4, 4, 4))
+
+
+def test_argument_errors_are_reported():
+    h = C.c_void_p()
+    assert gl.lib().glu_gfix_create(C.byref(h), (C.c_size_t * 4)(4, 4, 4, 4), 7, 0) == -1
+    assert b"gftype" in gl.lib().glu_gfix_last_error()
+    assert gl.footprint((32, 32, 32, 64)) > 32**3 * 64 * 576
+
+
+def test_oracle_is_not_linked_into_the_product():
+    out = os.popen(f"nm -D --undefined-only {gl.LIB_PATH}").read()
+    assert "or_" not in " ".join(l.split()[-1] for l in out.splitlines() if l.strip())
