@@ -423,6 +423,7 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
   for (int t = 0; t < T; t++) done[t] = active[t] ? 0 : 1;
   hipMemcpyAsync(sc.done, done.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
   hipMemcpyAsync(sc.iters, zero.data(), T * sizeof(long long), hipMemcpyHostToDevice, s);
+  hipStreamSynchronize(s);
   double2 *U = (double2 *)c->U64, *G = (double2 *)cs->G, *F = (double2 *)c->F, *sn = (double2 *)c->sn,
           *O = (double2 *)c->old;
   double *part = c->part;
@@ -488,16 +489,20 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     for (int t = 0; t < T; t++) any = any || active[t];
     if (!any) break;
     if (int rc = run_slices(c, method, active)) return rc;
-    hipMemcpy(th.data(), sc.theta, T * sizeof(double), hipMemcpyDeviceToHost);
-    hipMemcpy(it.data(), sc.iters, T * sizeof(long long), hipMemcpyDeviceToHost);
+    // every copy below is ordered on the context stream (a non-blocking
+    // stream does not wait for the null stream, nor the null stream for it)
+    hipMemcpyAsync(th.data(), sc.theta, T * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipMemcpyAsync(it.data(), sc.iters, T * sizeof(long long), hipMemcpyDeviceToHost, s);
+    hipStreamSynchronize(s);
     // unitarity test of the converged slices (reunitarises them)
     std::vector<double> devs(T, 0.0);
     for (int t = 0; t < T; t++) sel[t] = active[t] && th[t] < acc;
-    hipMemcpy(dsel, sel.data(), T * sizeof(int), hipMemcpyHostToDevice);
+    hipMemcpyAsync(dsel, sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
     hipLaunchKernelGGL(k_cadequate, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->part,
                        c->g, cs->nbs);
     std::vector<double> pm((size_t)cs->nbs * T);
-    hipMemcpy(pm.data(), c->part, pm.size() * sizeof(double), hipMemcpyDeviceToHost);
+    hipMemcpyAsync(pm.data(), c->part, pm.size() * sizeof(double), hipMemcpyDeviceToHost, s);
+    if (hipStreamSynchronize(s) != hipSuccess) return gfix_fail(-2, "unitarity check failed");
     for (int t = 0; t < T; t++)
       if (sel[t])
         for (long long b = 0; b < cs->nbs; b++) devs[t] = std::max(devs[t], pm[t * cs->nbs + b]);
@@ -528,9 +533,10 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     bool anyr = false;
     for (int t = 0; t < T; t++) anyr = anyr || restart[t];
     if (anyr) {
-      hipMemcpy(dsel, restart.data(), T * sizeof(int), hipMemcpyHostToDevice);
+      hipMemcpyAsync(dsel, restart.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
       hipLaunchKernelGGL(k_crandom, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->g,
                          restart_seed + (uint64_t)round);
+      hipStreamSynchronize(s);  // `restart` is host memory read by the copy above
     }
   }
   hipLaunchKernelGGL(k_cfinal, dim3(nbl(c->g.V)), dim3(kBlock), 0, s, (double2 *)c->U64, (const double2 *)cs->G,

@@ -1,4 +1,4 @@
-// fft_kernels.hip -- LDS-staged mixed-radix (2, 3, 4) complex FFT along
+// fft_kernels.hip -- LDS-staged mixed-radix (2, 3, 4, 5, 7) complex FFT along
 // one lattice axis, batched over every pencil of the 4 packed fields, with
 // the p^2 preconditioner fused into the last forward axis.
 //
@@ -35,6 +35,30 @@ template <typename T, int R> __device__ __forceinline__ void dft_small(cx<T> (&v
     v[2] = csub(a, c);
     v[1] = cadd(b, id);
     v[3] = csub(b, id);
+  } else if constexpr (R == 5 || R == 7) {
+    // plain O(R^2) DFT for the odd primes GLU's extents may contain
+    constexpr double c5[5] = {1.0, 0.30901699437494745, -0.8090169943749475, -0.8090169943749475,
+                              0.30901699437494745};
+    constexpr double s5[5] = {0.0, 0.9510565162951535, 0.5877852522924731, -0.5877852522924731,
+                              -0.9510565162951535};
+    constexpr double c7[7] = {1.0, 0.6234898018587336, -0.22252093395631434, -0.9009688679024191,
+                              -0.9009688679024191, -0.22252093395631434, 0.6234898018587336};
+    constexpr double s7[7] = {0.0, 0.7818314824680298, 0.9749279121818236, 0.43388373911755823,
+                              -0.43388373911755823, -0.9749279121818236, -0.7818314824680298};
+    cx<T> y[R];
+#pragma unroll
+    for (int m = 0; m < R; m++) {
+      cx<T> acc = v[0];
+#pragma unroll
+      for (int q = 1; q < R; q++) {
+        const int k = (q * m) % R;
+        const T c = (T)(R == 5 ? c5[k] : c7[k]), sn = (T)(sign * (R == 5 ? s5[k] : s7[k]));
+        acc = cadd(acc, cmul(v[q], mk<T>(c, sn)));
+      }
+      y[m] = acc;
+    }
+#pragma unroll
+    for (int m = 0; m < R; m++) v[m] = y[m];
   } else if constexpr (R == 3) {
     const T c1 = T(-0.5), s1 = T(sign) * T(0.86602540378443864676);
     const cx<T> a = v[0], b = v[1], c = v[2];
@@ -96,8 +120,12 @@ __device__ __forceinline__ void fft_run(vec2<T> *lds, int ldP, const FftAxis &ax
       fft_stage<T, 4>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
     else if (R == 2)
       fft_stage<T, 2>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
-    else
+    else if (R == 3)
       fft_stage<T, 3>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
+    else if (R == 5)
+      fft_stage<T, 5>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
+    else
+      fft_stage<T, 7>(lds, ldP, ax.N, ax.P, Ns, ax.tw, sign);
     Ns *= R;
   }
 }

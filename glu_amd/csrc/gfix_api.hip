@@ -36,17 +36,25 @@ constexpr long long kBatch = 8;        // iterations enqueued between host check
 
 int factor_radices(int N, int *rad) {
   int n = 0;
-  while (N % 4 == 0) {
+  while (N % 4 == 0 && n < 8) {
     rad[n++] = 4;
     N /= 4;
   }
-  if (N % 2 == 0) {
+  if (N % 2 == 0 && n < 8) {
     rad[n++] = 2;
     N /= 2;
   }
-  while (N % 3 == 0) {
+  while (N % 3 == 0 && n < 8) {
     rad[n++] = 3;
     N /= 3;
+  }
+  while (N % 5 == 0 && n < 8) {
+    rad[n++] = 5;
+    N /= 5;
+  }
+  while (N % 7 == 0 && n < 8) {
+    rad[n++] = 7;
+    N /= 7;
   }
   return N == 1 ? n : -1;
 }
@@ -144,7 +152,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
       ax.N = N;
       ax.nst = factor_radices(N, ax.rad);
       if (ax.nst < 0) {
-        rc = gfix_fail(-1, "lattice extent %d: only radices 2 and 3 are supported", N);
+        rc = gfix_fail(-1, "lattice extent %d: extents must factor into 2, 3, 5 and 7", N);
         goto bad;
       }
       ax.S = c->g.S[mu];
@@ -498,9 +506,14 @@ int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_it
       *theta = th32;
       return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : gfix_fail(-2, "sync failed");
     }
+    // the CG state is kept when the FP64 links agree with where the FP32
+    // phase stopped; otherwise the FP64 phase restarts from an SD step
+    double thl = 0.0;
+    if (int rc = theta_lin<double>(c, c->U64, &thl)) return rc;
+    const int keep = sw && 0.5 * thl < 10.0 * th32;
     size_t it64 = 0;
     const size_t left = max_iters > (size_t)tot ? max_iters - (size_t)tot : 1;
-    if (int rc = landau_run<double>(c, method, gf, acc, left, 0.0, sw, theta, &it64, &sw, &tot)) return rc;
+    if (int rc = landau_run<double>(c, method, gf, acc, left, 0.0, keep, theta, &it64, &sw, &tot)) return rc;
     c->last_fp64_iters = (size_t)tot;
     *iters = it64 >= 123456789 ? it64 : c->last_fp32_iters + c->last_fp64_iters;
     return 0;
@@ -522,7 +535,8 @@ extern "C" int glu_gfix_run(glu_gfix_ctx *c, int method, double gf_tune, double 
 extern "C" int glu_gfix_trace(glu_gfix_ctx *c, double *rows, size_t cap, size_t *n) {
   if (!c || !n) return gfix_fail(-1, "glu_gfix_trace: null argument");
   const size_t m = std::min<size_t>(cap, (size_t)c->trace_n);
-  if (m && hipMemcpy(rows, c->trace, m * 3 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+  if (m && (hipMemcpyAsync(rows, c->trace, m * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess))
     return gfix_fail(-2, "trace readback failed");
   *n = m;
   return 0;

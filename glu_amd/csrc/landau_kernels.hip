@@ -189,7 +189,11 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
   for (int mu = 0; mu < kND; mu++) {
     const su3<T> B = gauge_at<T>(F, V, nb_fwd(g, i, x, mu), (T)alpha);
     const su3<T> u = load_link<T>(U, mu, i, V);
-    store_link<T>(U, mu, i, V, gtransform<T>(A, u, B));
+    su3<T> n = gtransform<T>(A, u, B);
+    // FP32 links are kept on SU(3): unchecked, FP32 rounding of the columns
+    // feeds back through the completed third column over hundreds of updates
+    if (sizeof(T) == 4) gram_reunit<T>(n);
+    store_link<T>(U, mu, i, V, n);
   }
   if (G) {
     const su3<double> A64 = gauge_at64<T>(F, V, i, alpha);
