@@ -31,7 +31,7 @@ EXPORTS = [
     "glu_gfix_mixed_split", "glu_gfix_slice_info", "glu_gfix_stage_deriv", "glu_gfix_stage_fa",
     "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
-    "glu_gfix_profile", "glu_gfix_profile_read",
+    "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge"]
@@ -73,6 +73,7 @@ def lib():
             "glu_gfix_heatbath": (C.c_int, [P, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int]),
             "glu_gfix_random_transform": (C.c_int, [P, C.c_uint64]),
             "glu_gfix_profile": (C.c_int, [P, C.c_int]),
+            "glu_gfix_set_switch_theta": (C.c_int, [P, C.c_double]),
             "glu_gfix_profile_read": (C.c_int, [P, C.c_int, C.POINTER(SZ), DP, DP]),
         }
         for name, (res, args) in sig.items():
@@ -166,6 +167,9 @@ class Gfix:
     def reunit(self):
         _check(lib().glu_gfix_reunit(self._h), "reunit")
 
+    def set_switch_theta(self, theta: float):
+        _check(lib().glu_gfix_set_switch_theta(self._h, theta), "set_switch_theta")
+
     def mixed_split(self):
         a, b = SZ(), SZ()
         _check(lib().glu_gfix_mixed_split(self._h, C.byref(a), C.byref(b)), "mixed_split")
@@ -185,8 +189,10 @@ class Gfix:
     def random_transform(self, seed=1):
         _check(lib().glu_gfix_random_transform(self._h, seed), "random_transform")
 
-    def profile(self, enable=True):
-        _check(lib().glu_gfix_profile(self._h, int(enable)), "profile")
+    def profile(self, mask=-1):
+        """Time kernel classes with HIP events (bit k = KERNELS[k % 6], +6
+        for FP32); -1 all, 0 off."""
+        _check(lib().glu_gfix_profile(self._h, int(mask)), "profile")
 
     def profile_read(self):
         """{(kernel, 'fp64'|'fp32'): (launches, mean_ms, total_ms)}"""

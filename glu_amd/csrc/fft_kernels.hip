@@ -201,9 +201,182 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_axis(vec2<T> *__restrict__ 
   }
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time specialisation for the common extents: N, P and the radix
+// plan are constants, so every index, modulo and loop bound folds away.
+// Each thread keeps one pencil for the whole pass (P divides the block), the
+// twiddles sit in LDS, and the p^2 factor of the fused pass is the pencil's
+// cos sum plus one table entry per element.
+// ---------------------------------------------------------------------------
+struct RadPlan {
+  int n;
+  int r[8];
+};
+constexpr RadPlan make_plan(int N) {
+  RadPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  while (N % 4 == 0) {
+    p.r[p.n++] = 4;
+    N /= 4;
+  }
+  if (N % 2 == 0) {
+    p.r[p.n++] = 2;
+    N /= 2;
+  }
+  while (N % 3 == 0) {
+    p.r[p.n++] = 3;
+    N /= 3;
+  }
+  return p;
+}
+constexpr int fast_P(int N) {
+  int p = 1;
+  while (p * 2 <= 2048 / N && p * 2 <= 256) p *= 2;
+  return p;
+}
+
+template <typename T, int N, int P, int R, int Ns>
+__device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int sign) {
+  constexpr int nbf = N / R, total = nbf * P, LD = P + 1, tstep = N / (Ns * R);
+  constexpr int items = (total + kFftThreads - 1) / kFftThreads;
+  cx<T> v[items][R];
+#pragma unroll
+  for (int it = 0; it < items; it++) {
+    const int w = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads == 0 || w < total) {
+      const int j = w % P, b = w / P, k = b % Ns;
+#pragma unroll
+      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * LD + j);
+      if constexpr (Ns > 1) {
+#pragma unroll
+        for (int q = 1; q < R; q++) {
+          cx<T> t = twl[(q * k * tstep) % N];
+          if (sign > 0) t.im = -t.im;
+          v[it][q] = cmul(v[it][q], t);
+        }
+      }
+      dft_small<T, R>(v[it], sign);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < items; it++) {
+    const int w = threadIdx.x + it * kFftThreads;
+    if (total % kFftThreads == 0 || w < total) {
+      const int j = w % P, b = w / P, k = b % Ns;
+      const int out = (b / Ns) * Ns * R + k;
+#pragma unroll
+      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * LD + j, v[it][q]);
+    }
+  }
+  __syncthreads();
+}
+
+template <typename T, int N, int P, int S_, int Ns>
+__device__ __forceinline__ void fast_run(vec2<T> *lds, const cx<T> *twl, int sign) {
+  constexpr RadPlan pl = make_plan(N);
+  if constexpr (S_ < pl.n) {
+    constexpr int R = pl.r[S_];
+    fast_stage<T, N, P, R, Ns>(lds, twl, sign);
+    fast_run<T, N, P, S_ + 1, Ns * R>(lds, twl, sign);
+  }
+}
+
+template <typename T, int N>
+__global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ F, long long fstride, long long S,
+                                                         long long npencil, const double2 *__restrict__ tw, int mode,
+                                                         PsqInfo ps, const Ctrl *__restrict__ ctrl,
+                                                         long long slice_len, const int *__restrict__ slice_done) {
+  constexpr int P = fast_P(N), LD = P + 1, TOT = N * P;
+  static_assert(kFftThreads % P == 0, "a pencil per thread needs P | threads");
+  if (ctrl && ctrl->done) return;
+  if (slice_done && slice_done[blockIdx.z]) return;
+  __shared__ vec2<T> lds[N * LD];
+  __shared__ cx<T> twl[N];
+  for (int k = threadIdx.x; k < N; k += kFftThreads) twl[k] = mk<T>((T)tw[k].x, (T)tw[k].y);
+  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len;
+  const long long p0 = (long long)blockIdx.x * P;
+  // this thread's pencil and first element (strided axes)
+  const int jt = threadIdx.x % P, n0 = threadIdx.x / P;
+  constexpr int nstep = kFftThreads / P;
+  const long long pt = p0 + jt;
+  const bool valid = pt < npencil;
+  long long a0 = 0;
+  if (S != 1 && valid) a0 = (pt % S) + (pt / S) * S * N;
+  if (S == 1) {
+    const long long lim = (npencil - p0) * N;  // elements left in this chunk
+    const vec2<T> *src = base + p0 * N;
+    for (int idx = threadIdx.x; idx < TOT; idx += kFftThreads)
+      if (idx < lim) lds[(idx % N) * LD + idx / N] = src[idx];
+  } else if (valid) {
+#pragma unroll 4
+    for (int n = n0; n < N; n += nstep) lds[n * LD + jt] = base[a0 + (long long)n * S];
+  }
+  __syncthreads();
+  if (mode == 1)
+    fast_run<T, N, P, 0, 1>(lds, twl, +1);
+  else
+    fast_run<T, N, P, 0, 1>(lds, twl, -1);
+  if (mode == 2) {
+    // p^2 = 2 (D - sum cos) with p^2(0) := 1 (Geometry/geometry.c:170-186);
+    // the fused axis is the last of the D, so its cos enters the sum last
+    double kc = 0.0;
+    long long r = a0;
+    for (int mu = 0; mu < ps.D - 1; mu++) {
+      kc += ps.cosv[ps.off[mu] + (int)(r % ps.L[mu])];
+      r /= ps.L[mu];
+    }
+    const double *cl = ps.cosv + ps.off[ps.D - 1];
+    if (valid && S != 1) {
+#pragma unroll 4
+      for (int n = n0; n < N; n += nstep) {
+        const double kcos = kc + cl[n];
+        const double p2 = (kcos == (double)ps.D) ? 1.0 : 2.0 * ((double)ps.D - kcos);
+        const T f = (T)(ps.max / p2);
+        const cx<T> z = ld<T>(lds + n * LD + jt);
+        st<T>(lds + n * LD + jt, mk<T>(z.re * f, z.im * f));
+      }
+    }
+    __syncthreads();
+    fast_run<T, N, P, 0, 1>(lds, twl, +1);
+  }
+  if (S == 1) {
+    const long long lim = (npencil - p0) * N;
+    vec2<T> *dst = base + p0 * N;
+    for (int idx = threadIdx.x; idx < TOT; idx += kFftThreads)
+      if (idx < lim) dst[idx] = lds[(idx % N) * LD + idx / N];
+  } else if (valid) {
+#pragma unroll 4
+    for (int n = n0; n < N; n += nstep) base[a0 + (long long)n * S] = lds[n * LD + jt];
+  }
+}
+
+template <typename T, int N>
+static bool try_fast(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+  if (ax.N != N) return false;
+  constexpr int P = fast_P(N);
+  // the fused p^2 pass runs on a strided axis (it is always the last one)
+  if (mode == 2 && ax.S == 1) return false;
+  const unsigned nb = (unsigned)((ax.npencil + P - 1) / P);
+  hipLaunchKernelGGL((k_fft_fast<T, N>), dim3(nb, nfield, nslice), dim3(kFftThreads), 0, s, (vec2<T> *)F, fstride,
+                     ax.S, ax.npencil, ax.tw, mode, ps, ctrl, slice_len, slice_done);
+  return true;
+}
+
 template <typename T>
 void launch_fft_axis(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
                      const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+  if (try_fast<T, 4>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 6>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 8>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 12>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 16>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 24>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 32>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 48>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 64>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
+      try_fast<T, 96>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done))
+    return;
   const unsigned nb = (unsigned)((ax.npencil + ax.P - 1) / ax.P);
   const size_t sm = (size_t)ax.N * (ax.P + 1) * sizeof(vec2<T>);
   hipLaunchKernelGGL(k_fft_axis<T>, dim3(nb, nfield, nslice), dim3(kFftThreads), sm, s, (vec2<T> *)F, fstride, ax,

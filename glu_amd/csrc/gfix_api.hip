@@ -655,12 +655,12 @@ extern "C" int glu_gfix_stage_egauge(glu_gfix_ctx *c, const double *fields, doub
 // ---------------------------------------------------------------------------
 // per-kernel timing
 // ---------------------------------------------------------------------------
-extern "C" int glu_gfix_profile(glu_gfix_ctx *c, int enable) {
+extern "C" int glu_gfix_profile(glu_gfix_ctx *c, int mask) {
   if (!c) return gfix_fail(-1, "null context");
   hipStreamSynchronize(c->stream);
   prof_harvest(c);
-  c->prof = enable;
-  if (enable)
+  c->prof = mask;
+  if (mask)
     for (auto &v : c->prof_ms) v.clear();
   return 0;
 }
@@ -689,5 +689,11 @@ extern "C" int glu_gfix_profile_read(glu_gfix_ctx *c, int kernel, size_t *launch
   *launches = n;
   *total_ms = tot;
   *mean_ms = n ? tot / n : 0.0;
+  return 0;
+}
+
+extern "C" int glu_gfix_set_switch_theta(glu_gfix_ctx *c, double theta) {
+  if (!c || !(theta >= 0.0)) return gfix_fail(-1, "glu_gfix_set_switch_theta: bad argument");
+  c->switch_theta = theta;
   return 0;
 }

@@ -44,7 +44,8 @@ struct glu_gfix_ctx {
   void *stage = nullptr;  // upload / download staging
   size_t last_fp32_iters = 0, last_fp64_iters = 0;
   CoulombState *cs = nullptr;
-  // profiling: id = kernel class + PK_COUNT for the FP32 variants
+  // profiling: id = kernel class + PK_COUNT for the FP32 variants; prof is
+  // a bit mask of the ids being timed
   int prof = 0;
   std::vector<ProfPending> prof_pending;
   std::vector<hipEvent_t> prof_pool;
@@ -54,7 +55,7 @@ struct glu_gfix_ctx {
 // time one launch statement with HIP events on the context stream
 #define GFIX_PROF(c, id, stmt)                                       \
   do {                                                              \
-    if ((c)->prof) {                                                \
+    if ((c)->prof & (1 << (id))) {                                  \
       ProfPending p_{(id), prof_event(c), prof_event(c)};           \
       hipEventRecord(p_.a, (c)->stream);                            \
       stmt;                                                         \

@@ -143,6 +143,10 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 /* Reunitarise every device link (Geometry/random_config.c:43-56). */
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
 
+/* Mixed precision: run FP32 until theta < switch_theta, then FP64 (default
+   1e-7, or the GLU_GFIX_SWITCH_THETA environment variable at creation). */
+int glu_gfix_set_switch_theta(glu_gfix_ctx *ctx, double switch_theta);
+
 /* Iterations of the last mixed-precision run spent in FP32 / FP64. */
 int glu_gfix_mixed_split(glu_gfix_ctx *ctx, size_t *fp32_iters,
                          size_t *fp64_iters);
@@ -178,9 +182,10 @@ int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
 /* Per-kernel timing with HIP events on the context's stream.  Kernel
    classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
    update, 4 line search, 5 gauge update; +6 for the FP32 variants.
+   `mask` selects the classes timed (bit k = class k; -1 all, 0 off).
    profile_read reports the launches that did work (launches issued past
    convergence return at once and are left out), their mean and total. */
-int glu_gfix_profile(glu_gfix_ctx *ctx, int enable);
+int glu_gfix_profile(glu_gfix_ctx *ctx, int mask);
 int glu_gfix_profile_read(glu_gfix_ctx *ctx, int kernel, size_t *launches,
                           double *mean_ms, double *total_ms);
 
