@@ -32,6 +32,9 @@ EXPORTS = [
     "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
+    "glu_gfix_get_links_device", "glu_gfix_set_graph",
+    "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
+    "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge"]
@@ -75,6 +78,15 @@ def lib():
             "glu_gfix_profile": (C.c_int, [P, C.c_int]),
             "glu_gfix_set_switch_theta": (C.c_int, [P, C.c_double]),
             "glu_gfix_profile_read": (C.c_int, [P, C.c_int, C.POINTER(SZ), DP, DP]),
+            "glu_gfix_get_links_device": (C.c_int, [P, C.c_void_p]),
+            "glu_gfix_set_graph": (C.c_int, [P, C.c_int]),
+            "glu_gfix_server_create": (C.c_int, [C.POINTER(P), SZ, C.c_int]),
+            "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int]),
+            "glu_gfix_server_submit": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_int, C.c_int, C.c_double,
+                                                 C.c_double, SZ, C.c_void_p, C.POINTER(C.c_int)]),
+            "glu_gfix_server_wait": (C.c_int, [P, C.c_int, DP, C.POINTER(SZ), DP]),
+            "glu_gfix_server_stats": (C.c_int, [P, C.POINTER(SZ), DP, DP, C.POINTER(SZ)]),
+            "glu_gfix_server_destroy": (C.c_int, [P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -128,6 +140,12 @@ class Gfix:
 
     def set_links_device(self, ptr: int):
         _check(lib().glu_gfix_set_links_device(self._h, C.c_void_p(ptr)), "set_links_device")
+
+    def get_links_device(self, ptr: int):
+        _check(lib().glu_gfix_get_links_device(self._h, C.c_void_p(ptr)), "get_links_device")
+
+    def set_graph(self, on: bool):
+        _check(lib().glu_gfix_set_graph(self._h, int(bool(on))), "set_graph")
 
     def get_links(self) -> np.ndarray:
         U = np.empty((self.V, 4, 18))
@@ -234,3 +252,51 @@ class Gfix:
 
 def footprint(dims, gftype=LANDAU, precision=FP64) -> int:
     return int(lib().glu_gfix_footprint((SZ * 4)(*dims), gftype, precision))
+
+
+class Server:
+    """Serving loop (wraps glu_gfix_server).  Request buffers are device
+    pointers to GLU-layout links, fixed in place."""
+
+    def __init__(self, hbm_budget: int = 0, max_active: int = 8, switch_theta: float = 0.0,
+                 use_graph: bool = True):
+        h = C.c_void_p()
+        _check(lib().glu_gfix_server_create(C.byref(h), hbm_budget, max_active), "server_create")
+        self._h = h
+        _check(lib().glu_gfix_server_configure(h, switch_theta, int(bool(use_graph))), "server_configure")
+
+    def submit(self, dims, ptr: int, gftype=LANDAU, precision=MIXED, method=CG, gf_tune=0.08,
+               accuracy=1e-14, max_iters=1000) -> int:
+        t = C.c_int()
+        _check(lib().glu_gfix_server_submit(self._h, (SZ * 4)(*dims), gftype, precision, method, gf_tune,
+                                            accuracy, max_iters, C.c_void_p(ptr), C.byref(t)), "server_submit")
+        return t.value
+
+    def wait(self, ticket: int):
+        th, it, lat = C.c_double(), SZ(), C.c_double()
+        _check(lib().glu_gfix_server_wait(self._h, ticket, C.byref(th), C.byref(it), C.byref(lat)),
+               "server_wait")
+        return it.value, th.value, lat.value
+
+    def stats(self):
+        n, p50, p99, peak = SZ(), C.c_double(), C.c_double(), SZ()
+        _check(lib().glu_gfix_server_stats(self._h, C.byref(n), C.byref(p50), C.byref(p99), C.byref(peak)),
+               "server_stats")
+        return {"completed": n.value, "p50_ms": p50.value, "p99_ms": p99.value, "peak_bytes": peak.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().glu_gfix_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

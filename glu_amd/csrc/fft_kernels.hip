@@ -234,9 +234,10 @@ constexpr int fast_P(int N) {
   return p;
 }
 
-template <typename T, int N, int P, int R, int Ns>
+// element n of pencil j at lds[n * SN + j * SJ]
+template <typename T, int N, int P, int R, int Ns, int SN, int SJ>
 __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int sign) {
-  constexpr int nbf = N / R, total = nbf * P, LD = P + 1, tstep = N / (Ns * R);
+  constexpr int nbf = N / R, total = nbf * P, tstep = N / (Ns * R);
   constexpr int items = (total + kFftThreads - 1) / kFftThreads;
   cx<T> v[items][R];
 #pragma unroll
@@ -245,7 +246,7 @@ __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int s
     if (total % kFftThreads == 0 || w < total) {
       const int j = w % P, b = w / P, k = b % Ns;
 #pragma unroll
-      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * LD + j);
+      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * SN + j * SJ);
       if constexpr (Ns > 1) {
 #pragma unroll
         for (int q = 1; q < R; q++) {
@@ -265,21 +266,75 @@ __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int s
       const int j = w % P, b = w / P, k = b % Ns;
       const int out = (b / Ns) * Ns * R + k;
 #pragma unroll
-      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * LD + j, v[it][q]);
+      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * SN + j * SJ, v[it][q]);
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int P, int S_, int Ns>
+template <typename T, int N, int P, int S_, int Ns, int SN = P + 1, int SJ = 1>
 __device__ __forceinline__ void fast_run(vec2<T> *lds, const cx<T> *twl, int sign) {
   constexpr RadPlan pl = make_plan(N);
   if constexpr (S_ < pl.n) {
     constexpr int R = pl.r[S_];
-    fast_stage<T, N, P, R, Ns>(lds, twl, sign);
-    fast_run<T, N, P, S_ + 1, Ns * R>(lds, twl, sign);
+    fast_stage<T, N, P, R, Ns, SN, SJ>(lds, twl, sign);
+    fast_run<T, N, P, S_ + 1, Ns * R, SN, SJ>(lds, twl, sign);
   }
 }
+
+// both in-plane axes of an x-y plane in one pass: the plane (contiguous in
+// memory) is staged as lds[x * (NY+1) + y]; the x transform runs on pencils
+// y with stride NY+1 between points, the y transform on pencils x with
+// stride 1; NY+1 is odd, so both access patterns are bank-conflict free
+template <typename T, int NX, int NY>
+__global__ __launch_bounds__(kFftThreads) void k_fft_xy(vec2<T> *__restrict__ F, long long fstride,
+                                                       const double2 *__restrict__ twx,
+                                                       const double2 *__restrict__ twy, int mode,
+                                                       const Ctrl *__restrict__ ctrl, long long slice_len,
+                                                       const int *__restrict__ slice_done) {
+  constexpr int LD = NY + 1, NE = NX * NY;
+  if (ctrl && ctrl->done) return;
+  if (slice_done && slice_done[blockIdx.z]) return;
+  __shared__ vec2<T> lds[NX * LD];
+  __shared__ cx<T> tx[NX], ty[NY];
+  for (int k = threadIdx.x; k < NX; k += kFftThreads) tx[k] = mk<T>((T)twx[k].x, (T)twx[k].y);
+  for (int k = threadIdx.x; k < NY; k += kFftThreads) ty[k] = mk<T>((T)twy[k].x, (T)twy[k].y);
+  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len + (long long)blockIdx.x * NE;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NE; e += kFftThreads) lds[(e % NX) * LD + e / NX] = base[e];
+  __syncthreads();
+  const int sign = mode == 1 ? +1 : -1;
+  fast_run<T, NX, NY, 0, 1, LD, 1>(lds, tx, sign);
+  fast_run<T, NY, NX, 0, 1, 1, LD>(lds, ty, sign);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NE; e += kFftThreads) base[e] = lds[(e % NX) * LD + e / NX];
+}
+
+template <typename T, int N>
+static bool try_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
+                   const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
+                   const int *slice_done) {
+  if (ax.N != N || ay.N != N) return false;
+  if (sizeof(T) * 2 * N * (N + 1) > 48 * 1024) return false;
+  hipLaunchKernelGGL((k_fft_xy<T, N, N>), dim3((unsigned)nplane, nfield, nslice), dim3(kFftThreads), 0, s,
+                     (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
+  return true;
+}
+
+template <typename T>
+bool launch_fft_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
+                   const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
+                   const int *slice_done) {
+  return try_xy<T, 8>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
+         try_xy<T, 16>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
+         try_xy<T, 24>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
+         try_xy<T, 32>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
+         try_xy<T, 48>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done);
+}
+template bool launch_fft_xy<double>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
+                                    hipStream_t, long long, int, long long, const int *);
+template bool launch_fft_xy<float>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
+                                   hipStream_t, long long, int, long long, const int *);
 
 template <typename T, int N>
 __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ F, long long fstride, long long S,
@@ -287,6 +342,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ 
                                                          PsqInfo ps, const Ctrl *__restrict__ ctrl,
                                                          long long slice_len, const int *__restrict__ slice_done) {
   constexpr int P = fast_P(N), LD = P + 1, TOT = N * P;
+  static_assert(LD > 0, "");
   static_assert(kFftThreads % P == 0, "a pencil per thread needs P | threads");
   if (ctrl && ctrl->done) return;
   if (slice_done && slice_done[blockIdx.z]) return;

@@ -121,7 +121,10 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     }                                                                          \
   } while (0)
   {
+    CK(hipGetDevice(&c->device));
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    CK(hipEventCreateWithFlags(&c->run_ev, hipEventDisableTiming));
+    if (const char *e = getenv("GLU_GFIX_GRAPH")) c->use_graph = atoi(e);
     CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
     CK(hipMalloc(&c->sn, (size_t)V * 4 * sizeof(double2)));
@@ -196,6 +199,9 @@ extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   coulomb_free(c);
   prof_harvest(c);
   for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
+  for (hipGraphExec_t &gx : c->graph)
+    if (gx) hipGraphExecDestroy(gx);
+  if (c->run_ev) hipEventDestroy(c->run_ev);
   if (c->hctrl) hipHostFree(c->hctrl);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -233,6 +239,19 @@ extern "C" int glu_gfix_set_links_device(glu_gfix_ctx *c, const double *links_de
   if (!c || !links_dev) return gfix_fail(-1, "glu_gfix_set_links_device: null argument");
   launch_pack<double>(links_dev, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
   if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "pack failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_get_links_device(glu_gfix_ctx *c, double *links_dev) {
+  if (!c || !links_dev) return gfix_fail(-1, "glu_gfix_get_links_device: null argument");
+  launch_unpack<double>(c->U64, links_dev, c->g.V, 4, 0, c->g.V, c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "unpack failed");
+  return 0;
+}
+
+extern "C" int glu_gfix_set_graph(glu_gfix_ctx *c, int on) {
+  if (!c) return gfix_fail(-1, "null context");
+  c->use_graph = on ? 1 : 0;
   return 0;
 }
 
@@ -281,14 +300,24 @@ template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, co
   const long long V = c->g.V;
   const int ns = D == 4 ? 1 : c->g.L[3];
   const long long sl = D == 4 ? 0 : c->g.Lcu;
-  for (int a = 0; a < D - 1; a++)
+  // number of x-y planes per transform
+  const long long nplane = (D == 4 ? V : c->g.Lcu) / c->g.S[2];
+  auto xy = [&](int mode) {
+    bool ok = false;
+    GFIX_PROF(c, pk<T>(PK_FFT),
+              ok = c->use_xy && launch_fft_xy<T>(F, V, 4, c->ax[0], c->ax[1], mode, ctrl, c->stream, nplane, ns, sl,
+                                                  slice_done));
+    return ok;
+  };
+  const bool fused = xy(0);
+  for (int a = fused ? 2 : 0; a < D - 1; a++)
     GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 0, c->ps, ctrl, c->stream, ns, sl, slice_done));
   GFIX_PROF(c, pk<T>(PK_FFT),
             launch_fft_axis<T>(F, V, 4, c->ax[D - 1], 2, c->ps, ctrl, c->stream, ns, sl, slice_done));
-  for (int a = D - 2; a >= 0; a--)
+  for (int a = D - 2; a >= (fused ? 2 : 0); a--)
     GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 1, c->ps, ctrl, c->stream, ns, sl, slice_done));
+  if (fused) xy(1);
 }
-
 hipEvent_t prof_event(glu_gfix_ctx *c) {
   if (!c->prof_pool.empty()) {
     hipEvent_t e = c->prof_pool.back();
@@ -391,135 +420,202 @@ template <typename T> static int theta_lin(glu_gfix_ctx *c, void *U, double *out
   return 0;
 }
 
-// FACG() / FASD() of Gfix/FACG.c:341-446 on the phase buffers of type T.
-// switch_theta > 0: stop early once theta < switch_theta (mixed precision).
-// resume: continue the CG state already in the control block and fields
-// (no steepest-descent start) -- used after the FP32 -> FP64 switch.
-// Returns GLU's iteration count; *switched tells whether the stop was the
-// precision switch.
-template <typename T>
-static int landau_run(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double switch_theta,
-                      int resume, double *theta, size_t *iters, int *switched, long long *done_iters) {
-  const Bufs b = bufs_for<T>(c);
+// ---------------------------------------------------------------------------
+// FACG() / FASD() of Gfix/FACG.c:341-446 as a non-blocking state machine.
+// landau_begin() checks theta (FACG.c:349-353), initialises the control
+// block and enqueues the first batch; landau_advance() inspects a finished
+// batch and enqueues the next, or closes a segment: continuation below
+// 1e3 acc (FACG.c:372-378), the 123456789 sentinel, or -- mixed precision --
+// the FP32 -> FP64 switch.  The synchronous glu_gfix_run() and the serving
+// loop drive the same machine, so a request's result does not depend on how
+// it was driven.
+// ---------------------------------------------------------------------------
+namespace {
+enum { PH_FP32 = 0, PH_FP64 = 1 };
+enum { START_NONE = 0, START_SD = 1, START_PLAIN = 2 };
+}  // namespace
+
+// (re)initialise the control block for a phase; resume keeps the CG state
+static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, size_t max_iters) {
+  RunState &r = c->run;
   Ctrl &h = *c->hctrl;
   if (resume) {
-    if (int rc = read_ctrl(c)) return rc;
-    h.switch_theta = switch_theta;
     h.switched = 0;
     h.total = 0;
   } else {
     memset(&h, 0, sizeof(Ctrl));
-    h.acc = acc;
-    h.gf_alpha = gf;
+    h.acc = r.acc;
+    h.gf_alpha = r.gf;
     h.cg_tol = kCgTol;
-    h.switch_theta = switch_theta;
     h.trace_cap = c->trace_cap;
     h.trace_n = c->trace_n;
   }
+  h.switch_theta = switch_theta;
   h.max_iters = (long long)max_iters;
   h.done = 1;
-  if (int rc = write_ctrl(c)) return rc;
-  *switched = 0;
-  *done_iters = 0;
-  size_t out = 0;
-  bool first = true;
-  // GLU repeats continuation segments without bound (Gfix/FACG.c:363-379);
-  // a cap keeps a pathological input from spinning forever
-  for (int segment = 0; segment < 64; segment++) {
-    if (method == GLU_GFIX_CG && !(resume && first))
-      enq_sd_start<T>(c, b);
+  r.max_phase = max_iters;
+  r.segments = 0;
+  return write_ctrl(c);
+}
+
+template <typename T> static void enq_iterations(glu_gfix_ctx *c) {
+  const Bufs b = bufs_for<T>(c);
+  for (long long k = 0; k < kBatch; k++) {
+    if (c->run.method == GLU_GFIX_CG)
+      enq_cg_iter<T>(c, b);
     else
-      launch_begin_segment(c->ctrl, 0, c->stream);
-    first = false;
-    for (;;) {
-      for (long long k = 0; k < kBatch; k++) {
-        if (method == GLU_GFIX_CG)
-          enq_cg_iter<T>(c, b);
-        else
-          enq_sd_iter<T>(c, b);
-      }
-      if (int rc = read_ctrl(c)) return rc;
-      if (h.done) break;
-    }
-    *theta = h.theta;
-    const long long loc = h.iters;
-    *done_iters = h.total;
-    c->trace_n = h.trace_n;
-    if (h.switched) {
-      *switched = 1;
-      out = (size_t)loc;
-      break;
-    }
-    if (method == GLU_GFIX_CG) {
-      out = (size_t)loc;
-      if ((size_t)loc >= max_iters) {
-        if (h.theta < 1e3 * acc) {
-          fprintf(stdout, "[GF] Continuation run \n");
-          continue;
-        }
-        out = 123456789;
-      }
-    } else {
-      if ((size_t)loc >= max_iters) {
-        if (h.theta < 1e3 * acc) {
-          fprintf(stdout, "[GF] Continuation run \n");
-          continue;
-        }
-        out = 123456789 + (size_t)loc;
-      } else {
-        out = (size_t)loc;
-      }
-    }
-    break;
+      enq_sd_iter<T>(c, b);
   }
-  *iters = out;
+}
+
+// one batch of iterations, replayed from a captured graph when enabled
+template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
+  if (!c->use_graph || c->prof) {
+    enq_iterations<T>(c);
+    return 0;
+  }
+  const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1);
+  if (!c->graph[kind]) {
+    hipGraph_t gr = nullptr;
+    if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+      return gfix_fail(-2, "graph capture failed to start");
+    enq_iterations<T>(c);
+    if (hipStreamEndCapture(c->stream, &gr) != hipSuccess || !gr) return gfix_fail(-2, "graph capture failed");
+    const hipError_t e = hipGraphInstantiate(&c->graph[kind], gr, nullptr, nullptr, 0);
+    hipGraphDestroy(gr);
+    if (e != hipSuccess) return gfix_fail(-2, "graph instantiation failed: %s", hipGetErrorString(e));
+  }
+  return hipGraphLaunch(c->graph[kind], c->stream) == hipSuccess ? 0 : gfix_fail(-2, "graph launch failed");
+}
+
+static int enq_batch(glu_gfix_ctx *c) {
+  RunState &r = c->run;
+  const bool f32 = r.phase == PH_FP32;
+  if (r.start == START_SD) {
+    if (f32)
+      enq_sd_start<float>(c, bufs_for<float>(c));
+    else
+      enq_sd_start<double>(c, bufs_for<double>(c));
+  } else if (r.start == START_PLAIN) {
+    launch_begin_segment(c->ctrl, 0, c->stream);
+  }
+  r.start = START_NONE;
+  if (int rc = f32 ? enq_batch_body<float>(c) : enq_batch_body<double>(c)) return rc;
+  if (hipMemcpyAsync(c->hctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipEventRecord(r.ev, c->stream) != hipSuccess)
+    return gfix_fail(-2, "batch enqueue failed: %s", hipGetErrorString(hipGetLastError()));
+  r.inflight = 1;
   return 0;
 }
 
-int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters) {
-  double th0 = 0.0;
+int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters) {
+  RunState &r = c->run;
+  if (r.inflight) return gfix_fail(-1, "a run is still in flight on this context");
+  r = RunState{};
+  r.ev = c->run_ev;
+  r.method = method;
+  r.gf = gf;
+  r.acc = acc;
+  r.max_iters = max_iters;
   c->trace_n = 0;
   c->last_fp32_iters = c->last_fp64_iters = 0;
+  double th0 = 0.0;
   if (int rc = theta_lin<double>(c, c->U64, &th0)) return rc;
-  *theta = th0;
   if (th0 < acc) {  // Gfix/FACG.c:349-353
-    *iters = 0;
+    r.finished = 1;
+    r.theta = th0;
+    r.iters = 0;
     return 0;
   }
-  int sw = 0;
-  long long tot = 0;
-  if (c->precision == GLU_GFIX_MIXED && c->switch_theta > acc) {
+  r.mixed = c->precision == GLU_GFIX_MIXED && c->switch_theta > acc;
+  if (r.mixed) {
     // FP32 phase on a demoted copy; the site transform accumulates in FP64
     launch_demote(c->U64, c->U32, c->g.V * 24, c->stream);
     launch_identity<double>(c->G64, c->g.V, c->stream);
-    size_t it32 = 0;
-    double th32 = 0.0;
-    if (int rc = landau_run<float>(c, method, gf, acc, max_iters, c->switch_theta, 0, &th32, &it32, &sw, &tot))
-      return rc;
-    c->last_fp32_iters = (size_t)tot;
-    // replay the accumulated transform on the FP64 links, carry the CG state
+    r.phase = PH_FP32;
+    if (int rc = phase_ctrl_init(c, 0, c->switch_theta, max_iters)) return rc;
+  } else {
+    r.phase = PH_FP64;
+    if (int rc = phase_ctrl_init(c, 0, 0.0, max_iters)) return rc;
+  }
+  r.start = method == GLU_GFIX_CG ? START_SD : START_PLAIN;
+  return enq_batch(c);
+}
+
+// 1: finished; 2: a batch ended and the next one is enqueued; 0: the batch
+// in flight has not ended (never with block); < 0: error
+int landau_advance(glu_gfix_ctx *c, int block) {
+  RunState &r = c->run;
+  if (r.finished) return 1;
+  if (r.inflight) {
+    const hipError_t e = block ? hipEventSynchronize(r.ev) : hipEventQuery(r.ev);
+    if (e == hipErrorNotReady) return 0;
+    if (e != hipSuccess) return gfix_fail(-2, "iteration batch failed: %s", hipGetErrorString(e));
+    r.inflight = 0;
+    if (c->prof) prof_harvest(c);
+  }
+  Ctrl &h = *c->hctrl;
+  if (!h.done) {
+    if (int rc = enq_batch(c)) return rc;
+    return 2;
+  }
+  // a segment ended
+  c->trace_n = h.trace_n;
+  const double theta = h.theta;
+  const long long loc = h.iters;
+  if (h.switched) {
+    // FP32 -> FP64: replay the accumulated transform on the FP64 links and
+    // carry the CG state when the links agree with where FP32 stopped
+    c->last_fp32_iters = (size_t)h.total;
     launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
     launch_promote(c->sn32, c->sn, c->g.V * 4, c->stream);
     launch_promote(c->old32, c->old, c->g.V * 4, c->stream);
-    if (!sw && (it32 >= 123456789 || !(th32 > acc))) {  // finished (or failed) in FP32
-      *iters = it32;
-      *theta = th32;
-      return hipStreamSynchronize(c->stream) == hipSuccess ? 0 : gfix_fail(-2, "sync failed");
-    }
-    // the CG state is kept when the FP64 links agree with where the FP32
-    // phase stopped; otherwise the FP64 phase restarts from an SD step
     double thl = 0.0;
     if (int rc = theta_lin<double>(c, c->U64, &thl)) return rc;
-    const int keep = sw && 0.5 * thl < 10.0 * th32;
-    size_t it64 = 0;
-    const size_t left = max_iters > (size_t)tot ? max_iters - (size_t)tot : 1;
-    if (int rc = landau_run<double>(c, method, gf, acc, left, 0.0, keep, theta, &it64, &sw, &tot)) return rc;
-    c->last_fp64_iters = (size_t)tot;
-    *iters = it64 >= 123456789 ? it64 : c->last_fp32_iters + c->last_fp64_iters;
-    return 0;
+    const int keep = 0.5 * thl < 10.0 * theta;
+    const size_t left = r.max_iters > c->last_fp32_iters ? r.max_iters - c->last_fp32_iters : 1;
+    r.phase = PH_FP64;
+    if (int rc = phase_ctrl_init(c, keep, 0.0, left)) return rc;
+    r.start = keep ? START_PLAIN : (r.method == GLU_GFIX_CG ? START_SD : START_PLAIN);
+    if (int rc = enq_batch(c)) return rc;
+    return 2;
   }
-  if (int rc = landau_run<double>(c, method, gf, acc, max_iters, 0.0, 0, theta, iters, &sw, &tot)) return rc;
-  c->last_fp64_iters = (size_t)tot;
+  size_t out = (size_t)loc;
+  if ((size_t)loc >= r.max_phase) {
+    // GLU repeats continuation segments without bound (Gfix/FACG.c:363-379);
+    // a cap keeps a pathological input from spinning forever
+    if (theta < 1e3 * r.acc && r.segments < 63) {
+      fprintf(stdout, "[GF] Continuation run \n");
+      r.segments++;
+      r.start = r.method == GLU_GFIX_CG ? START_SD : START_PLAIN;
+      if (int rc = enq_batch(c)) return rc;
+      return 2;
+    }
+    out = r.method == GLU_GFIX_CG ? (size_t)123456789 : (size_t)123456789 + (size_t)loc;
+  }
+  if (r.phase == PH_FP32) {  // mixed run that stopped in FP32 (converged or failed)
+    c->last_fp32_iters = (size_t)h.total;
+    launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "sync failed");
+    r.iters = out;
+  } else {
+    c->last_fp64_iters = (size_t)h.total;
+    r.iters = (r.mixed && out < 123456789) ? c->last_fp32_iters + c->last_fp64_iters : out;
+  }
+  r.theta = theta;
+  r.finished = 1;
+  return 1;
+}
+
+int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters) {
+  if (int rc = landau_begin(c, method, gf, acc, max_iters)) return rc;
+  for (;;) {
+    const int st = landau_advance(c, 1);
+    if (st < 0) return st;
+    if (st == 1) break;
+  }
+  *theta = c->run.theta;
+  *iters = c->run.iters;
   return 0;
 }
 

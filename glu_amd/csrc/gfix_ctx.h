@@ -16,8 +16,21 @@ struct ProfPending {
   hipEvent_t a, b;
 };
 
+// state of a Landau run driven batch by batch (see landau_begin/advance)
+struct RunState {
+  int method = 0, phase = 0, start = 0, mixed = 0;
+  double gf = 0, acc = 0;
+  size_t max_iters = 0, max_phase = 0;
+  int segments = 0;
+  int inflight = 0, finished = 0;
+  hipEvent_t ev = nullptr;
+  size_t iters = 0;
+  double theta = 0.0;
+};
+
 struct glu_gfix_ctx {
   size_t dims[4] = {0, 0, 0, 0};
+  int device = 0;  // HIP device the context's memory lives on
   gfix::Geo g{};
   int gftype = 0;
   int precision = 0;
@@ -28,6 +41,11 @@ struct glu_gfix_ctx {
   // FP32 packed fields of the mixed schedule (in, sn, in_old)
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
   double switch_theta = 1e-7;
+  int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
+  int use_graph = 0;  // replay iteration batches from captured HIP graphs
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // CG/SD x FP64/FP32
+  hipEvent_t run_ev = nullptr;
+  RunState run;
   // packed fields (4 x V complex each): in, sn (search direction), in_old
   void *F = nullptr, *sn = nullptr, *old = nullptr;
   double *part = nullptr;  // per-block partial sums
@@ -76,3 +94,5 @@ int coulomb_alloc(glu_gfix_ctx *c);
 void coulomb_free(glu_gfix_ctx *c);
 int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters);
 int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters);
+int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters);
+int landau_advance(glu_gfix_ctx *c, int block);

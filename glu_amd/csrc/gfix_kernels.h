@@ -36,6 +36,13 @@ void launch_fft_axis(void *F, long long fstride, int nfield, const FftAxis &ax, 
                      const Ctrl *ctrl, hipStream_t s, int nslice = 1, long long slice_len = 0,
                      const int *slice_done = nullptr);
 
+// both in-plane axes (x, y) in one pass when the plane fits a workgroup;
+// false when this shape has no plane kernel (the caller runs axis passes)
+template <typename T>
+bool launch_fft_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
+                   const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
+                   const int *slice_done);
+
 template <typename T>
 void launch_deriv(const void *U, void *F, double *part, const Geo &g, const Ctrl *ctrl, hipStream_t s);
 template <typename T>

@@ -27,6 +27,27 @@ struct Rng {
   }
 };
 
+// Gram-Schmidt with plain divisions: the generator keeps its own arithmetic
+// so that its output does not move when the solver's reunitarisation is
+// tuned (the heatbath is chaotic: one ulp anywhere changes the config)
+__device__ void hb_reunit(su3<double> &m) {
+  double n = cnorm2(m.e[0]) + cnorm2(m.e[3]) + cnorm2(m.e[6]);
+  n = 1.0 / sqrt(n);
+  m.e[0] = cscale(m.e[0], n);
+  m.e[3] = cscale(m.e[3], n);
+  m.e[6] = cscale(m.e[6], n);
+  const cx<double> p = cadd(cadd(cmulc(m.e[1], m.e[0]), cmulc(m.e[4], m.e[3])), cmulc(m.e[7], m.e[6]));
+  m.e[1] = csub(m.e[1], cmul(p, m.e[0]));
+  m.e[4] = csub(m.e[4], cmul(p, m.e[3]));
+  m.e[7] = csub(m.e[7], cmul(p, m.e[6]));
+  double q = cnorm2(m.e[1]) + cnorm2(m.e[4]) + cnorm2(m.e[7]);
+  q = 1.0 / sqrt(q);
+  m.e[1] = cscale(m.e[1], q);
+  m.e[4] = cscale(m.e[4], q);
+  m.e[7] = cscale(m.e[7], q);
+  complete_col2(m);
+}
+
 // staple sum A with the local action Re tr(U_mu(x) A)
 __device__ su3<double> staple(const double2 *__restrict__ U, const Geo &g, long long i, const int x[4], int mu) {
   su3<double> A;
@@ -132,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_heatbath(double2 *__restrict__ U, Ge
     }
   }
   // rows p, q were rotated; rebuild an exact SU(3) matrix from columns 0, 1
-  gram_reunit<double>(u);
+  hb_reunit(u);
   store_link<double>(U, mu, i, g.V, u);
 }
 
@@ -157,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_random_gauge(double2 *__restrict__ G
     const double rad = sqrt(-2.0 * log(a));
     m.e[k] = mk<double>(rad * cos(6.283185307179586 * b), rad * sin(6.283185307179586 * b));
   }
-  gram_reunit<double>(m);
+  hb_reunit(m);
   store_link<double>(G, 0, i, V, m);
 }
 

@@ -150,17 +150,20 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
   if (i < V) {
     int x[4];
     coords(g, i, x);
-    const su3<double> A1 = gauge_at64<T>(F, V, i, 0.15);
-    const su3<double> A2 = gauge_at64<T>(F, V, i, 0.30);
-#pragma unroll
-    for (int mu = 0; mu < kND; mu++) {
-      const su3<double> u = promote<T>(load_link<T>(U, mu, i, V));
-      const long long f = nb_fwd(g, i, x, mu);
-      v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
-      const su3<double> B1 = gauge_at64<T>(F, V, f, 0.15);
-      v[1] += retrace_abc_dag<double>(A1, u, B1);
-      const su3<double> B2 = gauge_at64<T>(F, V, f, 0.30);
-      v[2] += retrace_abc_dag<double>(A2, u, B2);
+    // one probe at a time keeps a single probe matrix per site live (the
+    // link loads of the second probe hit the caches)
+#pragma unroll 1
+    for (int pr = 1; pr <= 2; pr++) {
+      const double al = pr == 1 ? 0.15 : 0.30;
+      const su3<double> A = gauge_at64<T>(F, V, i, al);
+      double acc = 0.0;
+#pragma unroll 1
+      for (int mu = 0; mu < kND; mu++) {
+        const su3<double> u = promote<T>(load_link<T>(U, mu, i, V));
+        if (pr == 1) v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
+        acc += retrace_abc_dag<double>(A, u, gauge_at64<T>(F, V, nb_fwd(g, i, x, mu), al));
+      }
+      v[pr] = acc;
     }
   }
   block_sum_store<3>(v, part + 3 * blockIdx.x);

@@ -1,0 +1,409 @@
+// serve.hip -- a serving loop for streams of gauge-fixing requests.
+//
+// GLU fixes one configuration per call (Wrappers/GF_wrap.c:179-212) and
+// checks beforehand that its temporaries fit in memory
+// (Utils/GLU_memcheck.c:72-95).  The serving loop keeps that admission rule
+// for a whole device: requests wait in a FIFO queue until the device memory
+// their context needs (glu_gfix_footprint) fits under a budget, then run
+// concurrently, each on its own context and stream.  One worker thread
+// drives every admitted Landau run batch by batch (landau_begin /
+// landau_advance), so a long run never holds up a short one
+// (continuous batching).  Contexts are pooled by (dims, gftype,
+// precision) and reused, and idle ones are evicted when a new shape needs
+// their memory.  A request's result does not depend on what it was served
+// with: every context owns its buffers and every reduction is
+// deterministic.
+//
+// Coulomb requests run to completion when admitted, on the worker thread
+// (the batched-slice driver is synchronous), which delays the polling of
+// the Landau runs in flight meanwhile but not their results.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/glu_gfix.h"
+#include "gfix_ctx.h"
+
+using Clock = std::chrono::steady_clock;
+using gfix::launch_pack;
+using gfix::launch_unpack;
+
+size_t gfix_footprint(const size_t dims[4], int gftype, int precision);
+
+namespace {
+
+enum ReqState { RQ_QUEUED = 0, RQ_RUNNING, RQ_DONE, RQ_FAILED };
+
+struct Request {
+  size_t dims[4];
+  int gftype, precision, method;
+  double gf, acc;
+  size_t max_iters;
+  double *links;  // device buffer, GLU layout double[V][4][18], fixed in place
+  size_t bytes;
+  Clock::time_point t_submit, t_done;
+  int state = RQ_QUEUED;  // changed under the server lock only
+  int copying = 0;        // worker-only: the output copy is in flight
+  int waited = 0;         // a caller is waiting for this ticket
+  double switch_theta = 0.0;
+  int use_graph = 1;
+  int rc = 0;
+  std::string err;
+  double theta = 0.0;
+  size_t iters = 0;
+  glu_gfix_ctx *ctx = nullptr;
+  hipEvent_t ev = nullptr;
+};
+
+struct Pooled {
+  glu_gfix_ctx *ctx;
+  size_t dims[4];
+  int gftype, precision;
+  size_t bytes;
+  bool busy;
+  unsigned long long last_use;
+};
+
+bool same_shape(const size_t a[4], const size_t b[4]) {
+  return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3];
+}
+
+}  // namespace
+
+struct glu_gfix_server {
+  int device = 0;
+  size_t budget = 0;
+  int max_active = 0;
+  double switch_theta = 0.0;  // 0: the context default
+  int use_graph = 1;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::map<int, Request> reqs;
+  std::deque<int> queue;
+  std::vector<Pooled> pool;
+  size_t used = 0, peak = 0;
+  unsigned long long tick = 0;
+  int next_ticket = 1;
+  bool stop = false;
+  std::vector<double> lat_ms;
+  std::thread worker;
+};
+
+namespace {
+
+// context for a request: an idle pooled one of the same shape, else a new
+// one once idle contexts of other shapes have made room.  Called with the
+// lock held; returns nullptr when the request has to wait.
+glu_gfix_ctx *acquire(glu_gfix_server *s, const Request &r, int *err) {
+  for (Pooled &p : s->pool)
+    if (!p.busy && same_shape(p.dims, r.dims) && p.gftype == r.gftype && p.precision == r.precision) {
+      p.busy = true;
+      p.last_use = ++s->tick;
+      return p.ctx;
+    }
+  while (s->used + r.bytes > s->budget) {  // evict the least recently used idle context
+    int victim = -1;
+    for (int i = 0; i < (int)s->pool.size(); i++)
+      if (!s->pool[i].busy && (victim < 0 || s->pool[i].last_use < s->pool[victim].last_use)) victim = i;
+    if (victim < 0) return nullptr;
+    glu_gfix_destroy(s->pool[victim].ctx);
+    s->used -= s->pool[victim].bytes;
+    s->pool.erase(s->pool.begin() + victim);
+  }
+  glu_gfix_ctx *c = nullptr;
+  *err = glu_gfix_create(&c, r.dims, r.gftype, r.precision);
+  if (*err) return nullptr;
+  s->pool.push_back(Pooled{c, {r.dims[0], r.dims[1], r.dims[2], r.dims[3]}, r.gftype, r.precision, r.bytes, true,
+                           ++s->tick});
+  s->used += r.bytes;
+  s->peak = std::max(s->peak, s->used);
+  return c;
+}
+
+void release(glu_gfix_server *s, glu_gfix_ctx *c, bool broken) {
+  for (int i = 0; i < (int)s->pool.size(); i++)
+    if (s->pool[i].ctx == c) {
+      if (broken) {
+        glu_gfix_destroy(c);
+        s->used -= s->pool[i].bytes;
+        s->pool.erase(s->pool.begin() + i);
+      } else {
+        s->pool[i].busy = false;
+      }
+      return;
+    }
+}
+
+struct Ended {
+  Request *r;
+  int rc;
+  std::string err;
+};
+
+// finish a request (lock held)
+void complete(glu_gfix_server *s, Request &r, int rc, const std::string &err) {
+  r.t_done = Clock::now();
+  if (rc) {
+    r.state = RQ_FAILED;
+    r.rc = rc;
+    r.err = err;
+  } else {
+    r.state = RQ_DONE;
+    s->lat_ms.push_back(std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count());
+  }
+  if (r.ev) {
+    (void)hipEventDestroy(r.ev);
+    r.ev = nullptr;
+  }
+  release(s, r.ctx, rc != 0);
+  r.ctx = nullptr;
+  s->cv_done.notify_all();
+}
+
+// start a request on its context: load the links, begin the run.  Returns
+// 0 and sets *finished when nothing is left to poll.
+int start(glu_gfix_server *s, Request &r, bool *finished) {
+  glu_gfix_ctx *c = r.ctx;
+  *finished = false;
+  if (hipEventCreateWithFlags(&r.ev, hipEventDisableTiming) != hipSuccess)
+    return gfix_fail(-2, "serving: event creation failed");
+  c->use_graph = r.use_graph;
+  if (r.switch_theta > 0.0) c->switch_theta = r.switch_theta;
+  launch_pack<double>(r.links, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
+  if (r.gftype == GLU_GFIX_COULOMB) {
+    if (int rc = coulomb_fix(c, r.method, r.gf, r.acc, r.max_iters, &r.theta, &r.iters)) return rc;
+    *finished = true;
+    return 0;
+  }
+  if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
+  *finished = c->run.finished != 0;
+  if (*finished) {
+    r.theta = c->run.theta;
+    r.iters = c->run.iters;
+  }
+  return 0;
+}
+
+// enqueue the copy of the fixed links into the request's buffer
+int copy_out(Request &r) {
+  glu_gfix_ctx *c = r.ctx;
+  launch_unpack<double>(c->U64, r.links, c->g.V, 4, 0, c->g.V, c->stream);
+  if (hipEventRecord(r.ev, c->stream) != hipSuccess) return gfix_fail(-2, "serving: event record failed");
+  r.copying = 1;
+  return 0;
+}
+
+void worker_main(glu_gfix_server *s) {
+  (void)hipSetDevice(s->device);
+  // requests this thread drives; std::map nodes stay put while other
+  // threads insert, so the pointers stay valid until a request completes
+  std::vector<Request *> active;
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    // admission: FIFO, as many as the budget and the slot count allow
+    std::vector<Request *> admitted;
+    while (!s->queue.empty() && (int)(active.size() + admitted.size()) < s->max_active) {
+      Request &r = s->reqs[s->queue.front()];
+      int err = 0;
+      glu_gfix_ctx *c = acquire(s, r, &err);
+      if (err) {  // creation failed: fail the request, keep serving
+        s->queue.pop_front();
+        r.t_done = Clock::now();
+        r.state = RQ_FAILED;
+        r.rc = err;
+        r.err = glu_gfix_last_error();
+        s->cv_done.notify_all();
+        continue;
+      }
+      if (!c) break;  // waits for memory
+      r.ctx = c;
+      r.state = RQ_RUNNING;
+      r.switch_theta = s->switch_theta;
+      r.use_graph = s->use_graph;
+      admitted.push_back(&r);
+      s->queue.pop_front();
+    }
+    if (active.empty() && admitted.empty()) {
+      if (s->stop && s->queue.empty()) break;
+      s->cv_work.wait(lk);
+      continue;
+    }
+    lk.unlock();
+    // device work happens without the lock; only this thread touches a
+    // running request
+    std::vector<Ended> ended;
+    auto fail = [&](Request *r, int rc) { ended.push_back({r, rc, glu_gfix_last_error()}); };
+    for (Request *r : admitted) {
+      bool fin = false;
+      int rc = start(s, *r, &fin);
+      if (!rc && fin) rc = copy_out(*r);
+      if (rc)
+        fail(r, rc);
+      else
+        active.push_back(r);
+    }
+    bool progressed = !admitted.empty();
+    for (Request *r : active) {
+      if (!r->copying) {
+        const int st = landau_advance(r->ctx, 0);
+        if (st < 0) {
+          fail(r, st);
+          continue;
+        }
+        if (st == 1) {
+          r->theta = r->ctx->run.theta;
+          r->iters = r->ctx->run.iters;
+          if (int rc = copy_out(*r)) {
+            fail(r, rc);
+            continue;
+          }
+        }
+        if (st != 0) progressed = true;
+      }
+      if (r->copying) {
+        const hipError_t e = hipEventQuery(r->ev);
+        if (e == hipSuccess) {
+          ended.push_back({r, 0, std::string()});
+        } else if (e != hipErrorNotReady) {
+          gfix_fail(-2, "serving: output copy failed: %s", hipGetErrorString(e));
+          fail(r, -2);
+        }
+      }
+    }
+    lk.lock();
+    for (Ended &e : ended) {
+      complete(s, *e.r, e.rc, e.err);
+      active.erase(std::find(active.begin(), active.end(), e.r));
+    }
+    if (ended.empty() && !progressed) {
+      // every run in flight is busy on the device: poll again shortly
+      s->cv_work.wait_for(lk, std::chrono::microseconds(50));
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int glu_gfix_server_create(glu_gfix_server **out, size_t hbm_budget, int max_active) {
+  if (!out) return gfix_fail(-1, "glu_gfix_server_create: null argument");
+  *out = nullptr;
+  if (max_active < 1) return gfix_fail(-1, "max_active must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return gfix_fail(-3, "no HIP device: the gauge-fixing library has no CPU path");
+  glu_gfix_server *s = new glu_gfix_server();
+  (void)hipGetDevice(&s->device);
+  if (hbm_budget == 0) {  // default: 90% of the memory free now
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+      delete s;
+      return gfix_fail(-2, "hipMemGetInfo failed");
+    }
+    hbm_budget = fr / 10 * 9;
+  }
+  s->budget = hbm_budget;
+  s->max_active = max_active;
+  s->worker = std::thread(worker_main, s);
+  *out = s;
+  return 0;
+}
+
+extern "C" int glu_gfix_server_configure(glu_gfix_server *s, double switch_theta, int use_graph) {
+  if (!s) return gfix_fail(-1, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->switch_theta = switch_theta;
+  s->use_graph = use_graph ? 1 : 0;
+  return 0;
+}
+
+extern "C" int glu_gfix_server_submit(glu_gfix_server *s, const size_t dims[4], int gftype, int precision,
+                                      int method, double gf_tune, double accuracy, size_t max_iters,
+                                      double *links_dev, int *ticket) {
+  if (!s || !dims || !links_dev || !ticket) return gfix_fail(-1, "glu_gfix_server_submit: null argument");
+  if (gftype != GLU_GFIX_LANDAU && gftype != GLU_GFIX_COULOMB) return gfix_fail(-1, "unknown gftype %d", gftype);
+  if (precision != GLU_GFIX_FP64 && precision != GLU_GFIX_MIXED)
+    return gfix_fail(-1, "unknown precision %d", precision);
+  if (method != GLU_GFIX_CG && method != GLU_GFIX_SD) return gfix_fail(-1, "unknown method %d", method);
+  if (max_iters == 0) return gfix_fail(-1, "max_iters must be positive");
+  Request r{};
+  memcpy(r.dims, dims, sizeof(r.dims));
+  r.gftype = gftype;
+  r.precision = precision;
+  r.method = method;
+  r.gf = gf_tune;
+  r.acc = accuracy;
+  r.max_iters = max_iters;
+  r.links = links_dev;
+  r.bytes = gfix_footprint(dims, gftype, precision);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (r.bytes > s->budget)
+    return gfix_fail(-4, "request needs %zu bytes of device memory, above the budget of %zu", r.bytes, s->budget);
+  if (s->stop) return gfix_fail(-1, "server is shutting down");
+  r.t_submit = Clock::now();
+  const int t = s->next_ticket++;
+  s->reqs.emplace(t, r);
+  s->queue.push_back(t);
+  *ticket = t;
+  s->cv_work.notify_one();
+  return 0;
+}
+
+extern "C" int glu_gfix_server_wait(glu_gfix_server *s, int ticket, double *theta, size_t *iters,
+                                    double *latency_ms) {
+  if (!s) return gfix_fail(-1, "null server");
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end()) return gfix_fail(-1, "unknown ticket %d", ticket);
+  if (it->second.waited) return gfix_fail(-1, "ticket %d is already being waited for", ticket);
+  it->second.waited = 1;
+  s->cv_done.wait(lk, [&] { return it->second.state == RQ_DONE || it->second.state == RQ_FAILED; });
+  const Request r = it->second;
+  s->reqs.erase(it);
+  if (r.state == RQ_FAILED) return gfix_fail(r.rc, "request %d failed: %s", ticket, r.err.c_str());
+  if (theta) *theta = r.theta;
+  if (iters) *iters = r.iters;
+  if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r.t_done - r.t_submit).count();
+  return 0;
+}
+
+extern "C" int glu_gfix_server_stats(glu_gfix_server *s, size_t *completed, double *p50_ms, double *p99_ms,
+                                     size_t *peak_bytes) {
+  if (!s) return gfix_fail(-1, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::vector<double> v = s->lat_ms;
+  std::sort(v.begin(), v.end());
+  // nearest-rank percentiles
+  auto pct = [&](double p) {
+    if (v.empty()) return 0.0;
+    size_t k = (size_t)std::ceil(p / 100.0 * (double)v.size());
+    return v[std::min(v.size(), std::max<size_t>(k, 1)) - 1];
+  };
+  if (completed) *completed = v.size();
+  if (p50_ms) *p50_ms = pct(50.0);
+  if (p99_ms) *p99_ms = pct(99.0);
+  if (peak_bytes) *peak_bytes = s->peak;
+  return 0;
+}
+
+extern "C" int glu_gfix_server_destroy(glu_gfix_server *s) {
+  if (!s) return 0;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    s->cv_work.notify_all();
+  }
+  s->worker.join();
+  for (Pooled &p : s->pool) glu_gfix_destroy(p.ctx);
+  for (auto &kv : s->reqs)
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+  delete s;
+  return 0;
+}

@@ -86,11 +86,26 @@ __device__ __forceinline__ void store_link(vec2<T> *__restrict__ U, int mu, size
   st<T>(b + 5 * V, m.e[7]);
 }
 
+// 1/sqrt(x): the hardware estimate refined by Newton steps (two in FP64,
+// one in FP32) to within an ulp of 1/sqrt(x), without the division
+__device__ __forceinline__ double rsqrt_nr(double x) {
+  double r = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  r = r * (1.5 - h * r * r);
+  r = r * (1.5 - h * r * r);
+  return r;
+}
+__device__ __forceinline__ float rsqrt_nr(float x) {
+  float r = __builtin_amdgcn_rsqf(x);
+  r = r * (1.5f - 0.5f * x * r * r);
+  return r;
+}
+
 // Gram-Schmidt on columns 0 and 1, column 2 completed:
 // Matrix_Ops/gramschmidt.c:73-165
 template <typename T> __device__ __forceinline__ void gram_reunit(su3<T> &m) {
   T n = cnorm2(m.e[0]) + cnorm2(m.e[3]) + cnorm2(m.e[6]);
-  n = T(1) / sqrt(n);
+  n = rsqrt_nr(n);
   m.e[0] = cscale(m.e[0], n);
   m.e[3] = cscale(m.e[3], n);
   m.e[6] = cscale(m.e[6], n);
@@ -100,7 +115,7 @@ template <typename T> __device__ __forceinline__ void gram_reunit(su3<T> &m) {
   m.e[4] = csub(m.e[4], cmul(p, m.e[3]));
   m.e[7] = csub(m.e[7], cmul(p, m.e[6]));
   T q = cnorm2(m.e[1]) + cnorm2(m.e[4]) + cnorm2(m.e[7]);
-  q = T(1) / sqrt(q);
+  q = rsqrt_nr(q);
   m.e[1] = cscale(m.e[1], q);
   m.e[4] = cscale(m.e[4], q);
   m.e[7] = cscale(m.e[7], q);

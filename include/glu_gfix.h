@@ -112,6 +112,7 @@ int glu_gfix_set_links(glu_gfix_ctx *ctx, const double *links);
 int glu_gfix_get_links(glu_gfix_ctx *ctx, double *links);
 /* Same, with the GLU-layout buffer already in device memory. */
 int glu_gfix_set_links_device(glu_gfix_ctx *ctx, const double *links_dev);
+int glu_gfix_get_links_device(glu_gfix_ctx *ctx, double *links_dev);
 
 /* Keep a device copy of the current links / restore it (device to device). */
 int glu_gfix_snapshot(glu_gfix_ctx *ctx);
@@ -188,6 +189,47 @@ int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
 int glu_gfix_profile(glu_gfix_ctx *ctx, int mask);
 int glu_gfix_profile_read(glu_gfix_ctx *ctx, int kernel, size_t *launches,
                           double *mean_ms, double *total_ms);
+
+/* Replay each batch of iterations from a HIP graph captured on first use
+   (on = 1) instead of launching its kernels one by one.  Results are
+   identical; only the launch cost changes.  Also set by the
+   GLU_GFIX_GRAPH environment variable at creation.  Ignored while
+   per-kernel profiling is on. */
+int glu_gfix_set_graph(glu_gfix_ctx *ctx, int on);
+
+/* Serving loop for streams of requests (GLU fixes one configuration per
+   call, Wrappers/GF_wrap.c:179-212; this keeps its memory check,
+   Utils/GLU_memcheck.c:72-95, as the admission rule for a device).
+   Requests queue FIFO and are admitted while the device memory their
+   contexts need (glu_gfix_footprint) fits under hbm_budget bytes (0: 90%
+   of the free memory) and fewer than max_active run.  Admitted requests run
+   concurrently, one context and stream each, driven batch by batch by one
+   worker thread, so short runs overtake long ones.  Contexts are pooled
+   and reused per shape.  A request's links are a device buffer in GLU's
+   layout double[V][4][18], fixed in place (FACG()/FASD()/Coulomb_FA()
+   semantics as glu_gfix_run); the caller keeps it alive until
+   glu_gfix_server_wait returns.  Results are bitwise those of glu_gfix_run
+   on the same input.  Latency runs from submit to the output copy's
+   completion. */
+typedef struct glu_gfix_server glu_gfix_server;
+int glu_gfix_server_create(glu_gfix_server **srv, size_t hbm_budget,
+                           int max_active);
+/* switch_theta for MIXED requests (0: the default); use_graph as
+   glu_gfix_set_graph (default on).  Applies to requests admitted later. */
+int glu_gfix_server_configure(glu_gfix_server *srv, double switch_theta,
+                              int use_graph);
+int glu_gfix_server_submit(glu_gfix_server *srv, const size_t dims[4],
+                           int gftype, int precision, int method,
+                           double gf_tune, double accuracy, size_t max_iters,
+                           double *links_dev, int *ticket);
+/* Blocks until the request ends; returns its error if it failed. */
+int glu_gfix_server_wait(glu_gfix_server *srv, int ticket, double *theta,
+                         size_t *iters, double *latency_ms);
+/* Completed requests, nearest-rank p50/p99 latency, peak bytes held. */
+int glu_gfix_server_stats(glu_gfix_server *srv, size_t *completed,
+                          double *p50_ms, double *p99_ms, size_t *peak_bytes);
+/* Finishes every submitted request, then frees the pooled contexts. */
+int glu_gfix_server_destroy(glu_gfix_server *srv);
 
 /* Bytes of device memory a context of these dims needs. */
 size_t glu_gfix_footprint(const size_t dims[4], int gftype, int precision);

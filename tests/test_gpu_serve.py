@@ -1,0 +1,115 @@
+"""Graph replay and the serving loop, through the C ABI.
+
+Both change only how the iteration kernels are launched and interleaved,
+never what they compute, so the bar is bitwise: a batch replayed from a
+captured HIP graph, and a request served among others, give exactly the
+links, theta and iteration count of the eager, stand-alone
+glu_gfix_run on the same input."""
+import numpy as np
+import pytest
+
+from gpu_common import heatbath_config
+
+pytestmark = pytest.mark.gpu
+
+gl = pytest.importorskip("glu_amd")
+torch = pytest.importorskip("torch")
+
+
+def alone(dims, U, gftype=gl.LANDAU, precision=gl.FP64, method=gl.CG, acc=1e-14, graph=False):
+    with gl.Gfix(dims, gftype, precision) as g:
+        g.set_graph(graph)
+        g.set_links(U)
+        it, th = g.run(method, 0.08, acc, 1000)
+        return g.get_links(), it, th
+
+
+@pytest.mark.parametrize("precision,method", [(gl.FP64, gl.CG), (gl.MIXED, gl.CG), (gl.FP64, gl.SD)])
+def test_graph_replays_match_eager(precision, method):
+    dims = (8, 8, 8, 16)
+    U, _ = heatbath_config(dims, sweeps=30)
+    acc = 1e-14 if method == gl.CG else 1e-10
+    W0, it0, th0 = alone(dims, U, precision=precision, method=method, acc=acc)
+    with gl.Gfix(dims, gl.LANDAU, precision) as g:
+        g.set_links(U)
+        g.snapshot()
+        for rep in range(3):
+            # graph run, then an eager run in between: each replay must
+            # reproduce the eager result
+            g.restore()
+            g.set_graph(True)
+            it, th = g.run(method, 0.08, acc, 1000)
+            assert (it, th) == (it0, th0), rep
+            assert np.array_equal(g.get_links(), W0), rep
+            g.restore()
+            g.set_graph(False)
+            it, th = g.run(method, 0.08, acc, 1000)
+            assert (it, th) == (it0, th0)
+            assert np.array_equal(g.get_links(), W0)
+
+
+REQUESTS = [  # dims, gftype, precision, method, input seed
+    ((8, 8, 8, 8), gl.LANDAU, gl.MIXED, gl.CG, 1),
+    ((8, 8, 8, 16), gl.LANDAU, gl.FP64, gl.CG, 2),
+    ((12, 12, 12, 12), gl.LANDAU, gl.MIXED, gl.CG, 3),
+    ((8, 8, 8, 8), gl.LANDAU, gl.FP64, gl.SD, 4),
+    ((6, 6, 6, 12), gl.COULOMB, gl.FP64, gl.CG, 5),
+    ((8, 8, 8, 16), gl.LANDAU, gl.MIXED, gl.CG, 6),
+    ((8, 8, 8, 8), gl.LANDAU, gl.MIXED, gl.CG, 7),
+]
+
+
+def inputs():
+    out = []
+    for dims, gftype, prec, meth, seed in REQUESTS:
+        U, _ = heatbath_config(dims, sweeps=20, seed=seed, gseed=seed + 100)
+        out.append(U)
+    return out
+
+
+def test_served_requests_match_standalone_runs():
+    Us = inputs()
+    ref = []
+    for (dims, gftype, prec, meth, _), U in zip(REQUESTS, Us):
+        acc = 1e-14 if meth == gl.CG else 1e-10
+        ref.append(alone(dims, U, gftype, prec, meth, acc))
+    bufs = [torch.from_numpy(np.ascontiguousarray(U)).cuda() for U in Us]
+    torch.cuda.synchronize()
+    with gl.Server(max_active=4) as srv:
+        tickets = []
+        for (dims, gftype, prec, meth, _), b in zip(REQUESTS, bufs):
+            acc = 1e-14 if meth == gl.CG else 1e-10
+            tickets.append(srv.submit(dims, b.data_ptr(), gftype, prec, meth, 0.08, acc, 1000))
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(REQUESTS)
+    assert 0 < st["p50_ms"] <= st["p99_ms"]
+    for k, ((W, it, th), (it2, th2, lat), b) in enumerate(zip(ref, res, bufs)):
+        assert (it2, th2) == (it, th), k
+        assert lat > 0
+        assert np.array_equal(b.cpu().numpy(), W), k
+
+
+def test_admission_respects_budget_and_rejects_oversize():
+    dims_big, dims_small = (12, 12, 12, 12), (8, 8, 8, 8)
+    fb = gl.footprint(dims_big, gl.LANDAU, gl.MIXED)
+    fs = gl.footprint(dims_small, gl.LANDAU, gl.MIXED)
+    budget = fb + fs // 2  # the big one runs alone; small ones two at a time
+    Ub, _ = heatbath_config(dims_big, sweeps=20, seed=3, gseed=103)
+    Us, _ = heatbath_config(dims_small, sweeps=20, seed=1, gseed=101)
+    Wb = alone(dims_big, Ub, precision=gl.MIXED)
+    Ws = alone(dims_small, Us, precision=gl.MIXED)
+    order = [dims_small, dims_big, dims_small, dims_small, dims_big]
+    bufs = [torch.from_numpy(np.ascontiguousarray(Ub if d == dims_big else Us)).cuda() for d in order]
+    torch.cuda.synchronize()
+    with gl.Server(hbm_budget=budget, max_active=8) as srv:
+        with pytest.raises(gl.GfixError):
+            srv.submit((16, 16, 16, 16), bufs[0].data_ptr())
+        tickets = [srv.submit(d, b.data_ptr(), precision=gl.MIXED) for d, b in zip(order, bufs)]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["peak_bytes"] <= budget
+    for d, b, (it, th, _) in zip(order, bufs, res):
+        W, it0, th0 = Wb if d == dims_big else Ws
+        assert (it, th) == (it0, th0)
+        assert np.array_equal(b.cpu().numpy(), W)
