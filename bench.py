@@ -1,6 +1,7 @@
 """Benchmark: Landau gauge fixing (FACG) of 32^3x64 configurations to
-theta < 1e-14 on one MI355X, mixed precision (FP32 streaming until
-theta < 1e-7, then FP64) against the plain FP64 path on the same input.
+theta < 1e-14 on one MI355X, mixed precision (FP32 streaming until theta
+falls below --switch-theta, then FP64) against the plain FP64 path on the
+same input.
 
 One step = restore the input links from their device copy + one complete
 gauge fixing to theta < 1e-14 (both inside the timed region).  The input is
@@ -8,7 +9,10 @@ a device heatbath config (beta = 6.0, cold start, 4 overrelaxation passes
 per sweep) with a seeded random gauge transform, generated once before the
 timed region.
 
-Prints one JSON line (see the keys below).
+Prints one JSON line (see the keys below).  Two further workloads print a
+line of their own: --workload coulomb (48^3x96 Coulomb FACG, the 96 time
+slices batched) and --workload serve (a Poisson stream of 64 Landau
+requests of mixed sizes through the serving loop).
 """
 from __future__ import annotations
 
@@ -46,7 +50,12 @@ def parse():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-iters", type=int, default=20,
+                    help="CPU baseline sample: k and 2k oracle iterations")
+    ap.add_argument("--workload", choices=("landau", "coulomb", "serve"), default="landau")
+    ap.add_argument("--requests", type=int, default=64)
+    ap.add_argument("--rate", type=float, default=2.0,
+                    help="serve: Poisson arrivals per second (about 70%% of the one-at-a-time capacity)")
     ap.add_argument("--switch-theta", type=float, default=1e-11,
                     help="mixed precision: FP32 until theta falls below this")
     return ap.parse_args()
@@ -105,6 +114,10 @@ def main():
         sys.exit(2)
     import glu_amd as gl
 
+    if a.workload == "coulomb":
+        return coulomb_main(a)
+    if a.workload == "serve":
+        return serve_main(a)
     dims = tuple(int(x) for x in a.dims.split(","))
     V = math.prod(dims)
     print(f"[bench] lattice {dims} V={V}: generating input "
@@ -206,6 +219,169 @@ def cpu_baseline(g, dims, iters_fp64, k):
                        f"{k} and {2 * k} iterations timed ({ts[0]:.1f}s, {ts[1]:.1f}s), "
                        f"{per_iter:.2f}s/iteration scaled to the {iters_fp64} iterations of "
                        "the full FP64 solve")}
+
+
+COULOMB_ALG = (576 + 144 + 64) + 4 * 128 + 128 + 256 + (576 + 144 + 64) + (144 + 64 + 144)
+
+
+def coulomb_main(a):
+    """48^3x96 Coulomb FACG, FP64: all 96 slice problems advance together
+    (one launch per kernel covers every slice still iterating)."""
+    import glu_amd as gl
+
+    dims = (48, 48, 48, 96) if a.dims == "32,32,32,64" else tuple(int(x) for x in a.dims.split(","))
+    V = math.prod(dims)
+    g = gl.Gfix(dims, gl.COULOMB, gl.FP64)
+    t = time.perf_counter()
+    g.heatbath(6.0, a.sweeps, 4, seed=a.seed, cold_start=True)
+    plaq = g.measure()["plaquette"]
+    g.random_transform(seed=a.seed + 1)
+    g.snapshot()
+    print(f"[bench] coulomb {dims}: input ready in {time.perf_counter() - t:.1f}s, plaquette {plaq:.12f}",
+          file=sys.stderr, flush=True)
+    for _ in range(max(a.warmup, 1)):
+        g.restore()
+        g.run(gl.CG, 0.09, 1e-14, 650)
+    g.measure()
+    lat, its = [], []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        g.restore()
+        it, th = g.run(gl.CG, 0.09, 1e-14, 650)
+        lat.append(time.perf_counter() - t)
+        its.append(it)
+        if not th < 1e-14:
+            raise RuntimeError(f"did not converge: theta {th}")
+    total = time.perf_counter() - t0
+    sits, sth = g.slice_info()
+    m = g.measure()
+    # each slice iteration streams one slice's share of the Landau-like
+    # step sequence; the batched launches cover the slices still running
+    gbs = sum(sits) * (V // dims[3]) * COULOMB_ALG / (sum(lat) / len(lat)) / 1e9
+    line = {
+        "metric": "Coulomb configs/s to theta<1e-14 at 48^3x96 (96 slices batched)",
+        "value": round(a.steps / total, 4), "unit": "configs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, "
+                 f"seed {a.seed}) + seeded random gauge transform"),
+        "config": {"workload": "coulomb_facg_fp64_" + "x".join(map(str, dims)), "lattice": list(dims),
+                   "gf_tune": 0.09, "accuracy": 1e-14, "max_iters": 650},
+        "p50_ms": round(1e3 * pct(lat, 0.5), 2), "total_iters": its,
+        "slice_iters": {"min": min(sits), "max": max(sits), "mean": round(sum(sits) / len(sits), 1)},
+        "max_slice_theta": float(max(sth)), "plaquette_drift": abs(m["plaquette"] - plaq),
+        "stream_GBps": round(gbs, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+SERVE_SIZES = ((16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64))
+SERVE_BETAS = (5.8, 6.0, 6.2)
+
+
+def serve_main(a):
+    """A stream of Landau requests (mixed precision, theta < 1e-14) with
+    Poisson arrivals through the serving loop.  Sizes and betas are drawn
+    uniformly; each request is a heatbath config of its (size, beta) with a
+    random gauge transform of its own, resident in device memory before
+    the stream starts.  The same arrival schedule is served twice: with
+    continuous batching (8 slots) and one request at a time."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import glu_amd as gl
+
+    rng = np.random.default_rng(a.seed)
+    n = a.requests
+    picks = [(SERVE_SIZES[rng.integers(3)], SERVE_BETAS[rng.integers(3)]) for _ in range(n)]
+    gaps = rng.exponential(1.0 / a.rate, size=n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    t = time.perf_counter()
+    base = {}
+    for dims in SERVE_SIZES:
+        for beta in SERVE_BETAS:
+            if (dims, beta) not in {p for p in picks}:
+                continue
+            g = gl.Gfix(dims, gl.LANDAU, gl.FP64)
+            g.heatbath(beta, a.sweeps, 4, seed=a.seed + int(10 * beta), cold_start=True)
+            g.snapshot()
+            base[(dims, beta)] = g
+    bufs = []
+    for k, (dims, beta) in enumerate(picks):
+        g = base[(dims, beta)]
+        g.restore()
+        g.random_transform(seed=a.seed + 1000 + k)
+        b = torch.empty(math.prod(dims) * 72, dtype=torch.float64, device="cuda")
+        g.get_links_device(b.data_ptr())
+        bufs.append(b)
+    inputs = [b.clone() for b in bufs]
+    for g in base.values():
+        g.close()
+    torch.cuda.synchronize()
+    print(f"[bench] serve: {n} requests ready in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+
+    def stream(slots):
+        for b, x in zip(bufs, inputs):
+            b.copy_(x)
+        torch.cuda.synchronize()
+        srv = gl.Server(max_active=slots, switch_theta=a.switch_theta)
+        # warm the context pool: one request of each size, untimed
+        for dims in SERVE_SIZES:
+            k = next((i for i, p in enumerate(picks) if p[0] == dims), None)
+            if k is not None:
+                srv.wait(srv.submit(dims, bufs[k].data_ptr(), precision=gl.MIXED))
+                bufs[k].copy_(inputs[k])
+        torch.cuda.synchronize()
+        tickets = [None] * n
+
+        def feeder(t0):
+            for k, (dims, _) in enumerate(picks):
+                dt = t0 + arrivals[k] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                tickets[k] = srv.submit(dims, bufs[k].data_ptr(), precision=gl.MIXED)
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=feeder, args=(t0,))
+        th.start()
+        th.join()
+        res = [srv.wait(tk) for tk in tickets]
+        wall = time.perf_counter() - t0
+        srv.close()
+        lats = [r[2] for r in res]
+        # a restarted request reports 123456789 + ..., as Landau() does
+        restarted = sum(1 for r in res if r[0] >= 123456789)
+        its = [r[0] for r in res if r[0] < 123456789]
+        if not all(r[1] < 1e-14 for r in res):
+            raise RuntimeError("a request did not converge")
+        return {"configs_per_s": round(n / wall, 3), "wall_s": round(wall, 2),
+                "p50_ms": round(pct(lats, 0.5), 1), "p99_ms": round(pct(lats, 0.99), 1),
+                "mean_iters": round(sum(its) / max(len(its), 1), 1), "restarted": restarted}
+
+    cb = stream(8)
+    print(f"[bench] serve continuous batching: {cb}", file=sys.stderr, flush=True)
+    fifo = stream(1)
+    print(f"[bench] serve one at a time: {fifo}", file=sys.stderr, flush=True)
+    mix = {}
+    for dims, _ in picks:
+        mix["x".join(map(str, dims))] = mix.get("x".join(map(str, dims)), 0) + 1
+    line = {
+        "metric": "Landau serving stream: configs/s and p50/p99 latency (64 requests, Poisson arrivals)",
+        "value": cb["configs_per_s"], "unit": "configs/s", "n_gpus": 1, "steps": n, "warmup": 3,
+        "ms_per_step": round(1e3 * cb["wall_s"] / n, 2), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": (f"synthetic: device heatbath ({a.sweeps} sweeps) per (size, beta), one random "
+                 f"gauge transform per request, seed {a.seed}"),
+        "config": {"workload": "landau_serve_stream", "requests": n, "rate_per_s": a.rate,
+                   "sizes": mix, "betas": list(SERVE_BETAS), "accuracy": 1e-14,
+                   "switch_theta": a.switch_theta, "slots": 8},
+        "p50_ms": cb["p50_ms"], "p99_ms": cb["p99_ms"], "continuous_batching": cb,
+        "one_at_a_time": fifo,
+    }
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -32,7 +32,7 @@ EXPORTS = [
     "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
-    "glu_gfix_get_links_device", "glu_gfix_set_graph",
+    "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy",
 ]
@@ -81,7 +81,9 @@ def lib():
             "glu_gfix_get_links_device": (C.c_int, [P, C.c_void_p]),
             "glu_gfix_set_graph": (C.c_int, [P, C.c_int]),
             "glu_gfix_server_create": (C.c_int, [C.POINTER(P), SZ, C.c_int]),
-            "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int]),
+            "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int, C.c_uint64]),
+            "glu_gfix_landau": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, C.c_uint64, DP,
+                                          C.POINTER(SZ), C.POINTER(C.c_int)]),
             "glu_gfix_server_submit": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_int, C.c_int, C.c_double,
                                                  C.c_double, SZ, C.c_void_p, C.POINTER(C.c_int)]),
             "glu_gfix_server_wait": (C.c_int, [P, C.c_int, DP, C.POINTER(SZ), DP]),
@@ -164,6 +166,14 @@ class Gfix:
         _check(lib().glu_gfix_run(self._h, method, gf_tune, accuracy, max_iters,
                                   C.byref(th), C.byref(it)), "run")
         return int(it.value), th.value
+
+    def landau(self, method=CG, gf_tune=0.08, accuracy=1e-14, max_iters=1000, restart_seed=1):
+        """Landau() semantics: run, with GLU's random restarts on failure.
+        Returns (iters, theta, restarts)."""
+        th, it, nf = C.c_double(), SZ(), C.c_int()
+        _check(lib().glu_gfix_landau(self._h, method, gf_tune, accuracy, max_iters, restart_seed,
+                                     C.byref(th), C.byref(it), C.byref(nf)), "landau")
+        return it.value, th.value, nf.value
 
     def trace(self, cap=16384) -> np.ndarray:
         rows = np.zeros((cap, 3))
@@ -259,11 +269,12 @@ class Server:
     pointers to GLU-layout links, fixed in place."""
 
     def __init__(self, hbm_budget: int = 0, max_active: int = 8, switch_theta: float = 0.0,
-                 use_graph: bool = True):
+                 use_graph: bool = True, restart_seed: int = 1):
         h = C.c_void_p()
         _check(lib().glu_gfix_server_create(C.byref(h), hbm_budget, max_active), "server_create")
         self._h = h
-        _check(lib().glu_gfix_server_configure(h, switch_theta, int(bool(use_graph))), "server_configure")
+        _check(lib().glu_gfix_server_configure(h, switch_theta, int(bool(use_graph)), restart_seed),
+               "server_configure")
 
     def submit(self, dims, ptr: int, gftype=LANDAU, precision=MIXED, method=CG, gf_tune=0.08,
                accuracy=1e-14, max_iters=1000) -> int:

@@ -520,7 +520,7 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
         if (th[t] > acc || !std::isfinite(th[t])) trans = true;
       }
       if (control[t] < 8 && trans) {
-        fprintf(stdout, "[CG] random gauge transform %d :: %e (%zu)\n", control[t], th[t], loc);
+        if (c->verbose) fprintf(stdout, "[CG] random gauge transform %d :: %e (%zu)\n", control[t], th[t], loc);
         restart[t] = 1;
         tot[t] += loc;
         control[t]++;

@@ -33,6 +33,7 @@ namespace {
 constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-455
 constexpr double kSwitchTheta = 1e-7;  // mixed precision: FP32 until theta < 1e-7
 constexpr long long kBatch = 8;        // iterations enqueued between host checks
+constexpr int kGluFailures = 7;        // GF_GLU_FAILURES, Headers/GLU_definitions.h:400-402
 
 int factor_radices(int N, int *rad) {
   int n = 0;
@@ -192,7 +193,7 @@ bad:
 extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *bufs[] = {c->U64, c->U32, c->G64, c->Usnap, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
+  void *bufs[] = {c->U64, c->U32, c->G64, c->Usnap, c->Uorig, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
                   c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -585,7 +586,7 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     // GLU repeats continuation segments without bound (Gfix/FACG.c:363-379);
     // a cap keeps a pathological input from spinning forever
     if (theta < 1e3 * r.acc && r.segments < 63) {
-      fprintf(stdout, "[GF] Continuation run \n");
+      if (c->verbose) fprintf(stdout, "[GF] Continuation run \n");
       r.segments++;
       r.start = r.method == GLU_GFIX_CG ? START_SD : START_PLAIN;
       if (int rc = enq_batch(c)) return rc;
@@ -616,6 +617,56 @@ int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_it
   }
   *theta = c->run.theta;
   *iters = c->run.iters;
+  return 0;
+}
+
+// The restart policy of Landau(), Gfix/Landau.c:301-345: a run returning
+// exactly 123456789 (an FACG failure; FASD's 123456789 + loc does not
+// qualify, as in GLU) is repeated from the input links after a random gauge
+// transform, at most GF_GLU_FAILURES - 1 = 6 times
+// (Headers/GLU_definitions.h:400-402).  Failed attempts add max_iters to
+// the count, the successful one its own iterations, so a restarted run
+// reports 123456789 + ... as GLU does.  GLU re-reads the config file and
+// draws the transform from its thread-dependent MWC streams; here the input
+// comes from device memory and restart k uses the device transform seeded
+// restart_seed + k.  *failures: restarts taken (7 = gave up).
+int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters) {
+  if (loc != 123456789) {
+    *iters = first + loc;
+    return 1;
+  }
+  *iters = first + max_iters;
+  return 0;
+}
+
+extern "C" int glu_gfix_landau(glu_gfix_ctx *c, int method, double gf_tune, double accuracy, size_t max_iters,
+                               uint64_t restart_seed, double *theta, size_t *iters, int *failures) {
+  if (!c || !theta || !iters) return gfix_fail(-1, "glu_gfix_landau: null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "glu_gfix_landau: not a Landau context");
+  if (method != GLU_GFIX_CG && method != GLU_GFIX_SD) return gfix_fail(-1, "unknown method %d", method);
+  if (max_iters == 0) return gfix_fail(-1, "max_iters must be positive");
+  const size_t bytes = (size_t)c->g.V * 24 * sizeof(double2);
+  if (!c->Uorig && hipMalloc(&c->Uorig, bytes) != hipSuccess) return gfix_fail(-2, "input copy allocation failed");
+  if (hipMemcpyAsync(c->Uorig, c->U64, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    return gfix_fail(-2, "input copy failed");
+  size_t it = 0;
+  if (int rc = landau_fix(c, method, gf_tune, accuracy, max_iters, theta, &it)) return rc;
+  int fail = 0;
+  if (it == 123456789) {
+    size_t total = it;
+    for (fail = 1; fail < kGluFailures; fail++) {
+      if (hipMemcpyAsync(c->U64, c->Uorig, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        return gfix_fail(-2, "input reload failed");
+      if (int rc = glu_gfix_random_transform(c, restart_seed + (uint64_t)fail)) return rc;
+      size_t loc = 0;
+      if (int rc = landau_fix(c, method, gf_tune, accuracy, max_iters, theta, &loc)) return rc;
+      if (landau_restart_policy(total, loc, max_iters, &total)) break;
+    }
+    it = total;
+  }
+  if (failures) *failures = fail;
+  *iters = it;
+  if (fail == kGluFailures) return gfix_fail(-5, "no convergence after %d random restarts (GF_TUNE %g)", fail - 1, gf_tune);
   return 0;
 }
 

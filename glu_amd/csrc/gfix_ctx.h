@@ -30,7 +30,8 @@ struct RunState {
 
 struct glu_gfix_ctx {
   size_t dims[4] = {0, 0, 0, 0};
-  int device = 0;  // HIP device the context's memory lives on
+  int device = 0;   // HIP device the context's memory lives on
+  int verbose = 0;  // print GLU's progress lines (the drop-in entry points)
   gfix::Geo g{};
   int gftype = 0;
   int precision = 0;
@@ -38,6 +39,7 @@ struct glu_gfix_ctx {
   // links: FP64 column planes (4 dirs x 6 entries x V), FP32 copy and the
   // FP32 accumulated site transform for the mixed schedule, snapshot
   void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
+  void *Uorig = nullptr;  // input links kept for the Landau() restarts
   // FP32 packed fields of the mixed schedule (in, sn, in_old)
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
   double switch_theta = 1e-7;
@@ -96,3 +98,5 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
 int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters, double *theta, size_t *iters);
 int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters);
 int landau_advance(glu_gfix_ctx *c, int block);
+int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters);
+extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed);

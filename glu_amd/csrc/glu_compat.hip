@@ -48,6 +48,7 @@ glu_gfix_ctx *get_ctx(int gftype) {
     g_ctx = nullptr;
     return nullptr;
   }
+  g_ctx->verbose = 1;  // GLU's [GF] / [CG] progress lines
   memcpy(g_dims, Latt.dims, sizeof(g_dims));
   g_type = gftype;
   g_prec = prec;

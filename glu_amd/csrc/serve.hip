@@ -14,6 +14,7 @@
 // with: every context owns its buffers and every reduction is
 // deterministic.
 //
+// Landau requests follow Landau()'s restart policy (glu_gfix_landau);
 // Coulomb requests run to completion when admitted, on the worker thread
 // (the batched-slice driver is synchronous), which delays the polling of
 // the Landau runs in flight meanwhile but not their results.
@@ -55,6 +56,9 @@ struct Request {
   int waited = 0;         // a caller is waiting for this ticket
   double switch_theta = 0.0;
   int use_graph = 1;
+  uint64_t restart_seed = 0;
+  int attempt = 0;   // Landau() restarts taken so far
+  size_t total = 0;  // iterations counted across restarts
   int rc = 0;
   std::string err;
   double theta = 0.0;
@@ -84,6 +88,7 @@ struct glu_gfix_server {
   int max_active = 0;
   double switch_theta = 0.0;  // 0: the context default
   int use_graph = 1;
+  uint64_t restart_seed = 1;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::map<int, Request> reqs;
@@ -168,6 +173,8 @@ void complete(glu_gfix_server *s, Request &r, int rc, const std::string &err) {
   s->cv_done.notify_all();
 }
 
+int after_run(Request &r);
+
 // start a request on its context: load the links, begin the run.  Returns
 // 0 and sets *finished when nothing is left to poll.
 int start(glu_gfix_server *s, Request &r, bool *finished) {
@@ -184,12 +191,39 @@ int start(glu_gfix_server *s, Request &r, bool *finished) {
     return 0;
   }
   if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
-  *finished = c->run.finished != 0;
-  if (*finished) {
-    r.theta = c->run.theta;
-    r.iters = c->run.iters;
+  if (c->run.finished) {
+    const int more = after_run(r);
+    if (more < 0) return more;
+    *finished = more == 0;
   }
   return 0;
+}
+
+// a Landau run ended: apply the restart policy of Landau()
+// (Gfix/Landau.c:301-345, as glu_gfix_landau).  0: the request is
+// finished, 1: a restarted run is in flight, < 0: error
+int after_run(Request &r) {
+  glu_gfix_ctx *c = r.ctx;
+  const size_t loc = c->run.iters;
+  r.theta = c->run.theta;
+  if (r.attempt == 0) {
+    r.total = loc;
+    if (loc != 123456789) {
+      r.iters = loc;
+      return 0;
+    }
+  } else if (landau_restart_policy(r.total, loc, r.max_iters, &r.total)) {
+    r.iters = r.total;
+    return 0;
+  }
+  if (++r.attempt >= 7)
+    return gfix_fail(-5, "no convergence after %d random restarts (GF_TUNE %g)", r.attempt - 1, r.gf);
+  // restart from the request's input, which stays untouched until copy_out
+  launch_pack<double>(r.links, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
+  if (int rc = glu_gfix_random_transform(c, r.restart_seed + (uint64_t)r.attempt)) return rc;
+  if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
+  if (c->run.finished) return after_run(r);
+  return 1;
 }
 
 // enqueue the copy of the fixed links into the request's buffer
@@ -228,6 +262,7 @@ void worker_main(glu_gfix_server *s) {
       r.state = RQ_RUNNING;
       r.switch_theta = s->switch_theta;
       r.use_graph = s->use_graph;
+      r.restart_seed = s->restart_seed;
       admitted.push_back(&r);
       s->queue.pop_front();
     }
@@ -259,11 +294,16 @@ void worker_main(glu_gfix_server *s) {
           continue;
         }
         if (st == 1) {
-          r->theta = r->ctx->run.theta;
-          r->iters = r->ctx->run.iters;
-          if (int rc = copy_out(*r)) {
-            fail(r, rc);
+          const int more = after_run(*r);
+          if (more < 0) {
+            fail(r, more);
             continue;
+          }
+          if (more == 0) {
+            if (int rc = copy_out(*r)) {
+              fail(r, rc);
+              continue;
+            }
           }
         }
         if (st != 0) progressed = true;
@@ -316,11 +356,13 @@ extern "C" int glu_gfix_server_create(glu_gfix_server **out, size_t hbm_budget, 
   return 0;
 }
 
-extern "C" int glu_gfix_server_configure(glu_gfix_server *s, double switch_theta, int use_graph) {
+extern "C" int glu_gfix_server_configure(glu_gfix_server *s, double switch_theta, int use_graph,
+                                         uint64_t restart_seed) {
   if (!s) return gfix_fail(-1, "null server");
   std::lock_guard<std::mutex> lk(s->mu);
   s->switch_theta = switch_theta;
   s->use_graph = use_graph ? 1 : 0;
+  s->restart_seed = restart_seed;
   return 0;
 }
 

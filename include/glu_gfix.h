@@ -127,6 +127,19 @@ int glu_gfix_run(glu_gfix_ctx *ctx, int method, double gf_tune,
                  double accuracy, size_t max_iters, double *theta,
                  size_t *iters);
 
+/* Landau() of Gfix/Landau.c:261-374 on the device links: glu_gfix_run
+   with GLU's restart policy.  A run returning exactly 123456789 (FACG
+   failure) is repeated from the input links after a random gauge transform,
+   at most 6 times (GF_GLU_FAILURES = 7, Headers/GLU_definitions.h:400-402);
+   failed attempts add max_iters to *iters, as GLU counts them.  GLU
+   re-reads the config file and uses its own generator for the transform;
+   here restart k applies glu_gfix_random_transform(ctx, restart_seed + k)
+   to a device copy of the input.  *failures (may be NULL): restarts taken.
+   Returns -5 when all 7 attempts fail (GLU: GLU_FAILURE). */
+int glu_gfix_landau(glu_gfix_ctx *ctx, int method, double gf_tune,
+                    double accuracy, size_t max_iters, uint64_t restart_seed,
+                    double *theta, size_t *iters, int *failures);
+
 /* Trajectory of the last run: rows (theta_k, alpha_k, beta_k); row 0 is the
    steepest-descent start.  Landau only.  Recording costs nothing extra. */
 int glu_gfix_trace(glu_gfix_ctx *ctx, double *rows, size_t cap, size_t *n);
@@ -206,18 +219,19 @@ int glu_gfix_set_graph(glu_gfix_ctx *ctx, int on);
    concurrently, one context and stream each, driven batch by batch by one
    worker thread, so short runs overtake long ones.  Contexts are pooled
    and reused per shape.  A request's links are a device buffer in GLU's
-   layout double[V][4][18], fixed in place (FACG()/FASD()/Coulomb_FA()
-   semantics as glu_gfix_run); the caller keeps it alive until
-   glu_gfix_server_wait returns.  Results are bitwise those of glu_gfix_run
-   on the same input.  Latency runs from submit to the output copy's
+   layout double[V][4][18], fixed in place (Landau requests: glu_gfix_landau
+   semantics, restarts included; Coulomb: glu_gfix_run); the caller keeps it
+   alive until glu_gfix_server_wait returns.  Results are bitwise those of
+   the same call on a context of their own.  Latency runs from submit to the output copy's
    completion. */
 typedef struct glu_gfix_server glu_gfix_server;
 int glu_gfix_server_create(glu_gfix_server **srv, size_t hbm_budget,
                            int max_active);
 /* switch_theta for MIXED requests (0: the default); use_graph as
-   glu_gfix_set_graph (default on).  Applies to requests admitted later. */
+   glu_gfix_set_graph (default on); restart_seed as glu_gfix_landau
+   (default 1).  Applies to requests admitted later. */
 int glu_gfix_server_configure(glu_gfix_server *srv, double switch_theta,
-                              int use_graph);
+                              int use_graph, uint64_t restart_seed);
 int glu_gfix_server_submit(glu_gfix_server *srv, const size_t dims[4],
                            int gftype, int precision, int method,
                            double gf_tune, double accuracy, size_t max_iters,

@@ -16,11 +16,15 @@ gl = pytest.importorskip("glu_amd")
 torch = pytest.importorskip("torch")
 
 
-def alone(dims, U, gftype=gl.LANDAU, precision=gl.FP64, method=gl.CG, acc=1e-14, graph=False):
+def alone(dims, U, gftype=gl.LANDAU, precision=gl.FP64, method=gl.CG, acc=1e-14, graph=False,
+          landau=False):
     with gl.Gfix(dims, gftype, precision) as g:
         g.set_graph(graph)
         g.set_links(U)
-        it, th = g.run(method, 0.08, acc, 1000)
+        if landau and gftype == gl.LANDAU:
+            it, th, _ = g.landau(method, 0.08, acc, 1000, restart_seed=1)
+        else:
+            it, th = g.run(method, 0.08, acc, 1000)
         return g.get_links(), it, th
 
 
@@ -72,7 +76,7 @@ def test_served_requests_match_standalone_runs():
     ref = []
     for (dims, gftype, prec, meth, _), U in zip(REQUESTS, Us):
         acc = 1e-14 if meth == gl.CG else 1e-10
-        ref.append(alone(dims, U, gftype, prec, meth, acc))
+        ref.append(alone(dims, U, gftype, prec, meth, acc, landau=True))
     bufs = [torch.from_numpy(np.ascontiguousarray(U)).cuda() for U in Us]
     torch.cuda.synchronize()
     with gl.Server(max_active=4) as srv:
@@ -97,8 +101,8 @@ def test_admission_respects_budget_and_rejects_oversize():
     budget = fb + fs // 2  # the big one runs alone; small ones two at a time
     Ub, _ = heatbath_config(dims_big, sweeps=20, seed=3, gseed=103)
     Us, _ = heatbath_config(dims_small, sweeps=20, seed=1, gseed=101)
-    Wb = alone(dims_big, Ub, precision=gl.MIXED)
-    Ws = alone(dims_small, Us, precision=gl.MIXED)
+    Wb = alone(dims_big, Ub, precision=gl.MIXED, landau=True)
+    Ws = alone(dims_small, Us, precision=gl.MIXED, landau=True)
     order = [dims_small, dims_big, dims_small, dims_small, dims_big]
     bufs = [torch.from_numpy(np.ascontiguousarray(Ub if d == dims_big else Us)).cuda() for d in order]
     torch.cuda.synchronize()
@@ -113,3 +117,60 @@ def test_admission_respects_budget_and_rejects_oversize():
         W, it0, th0 = Wb if d == dims_big else Ws
         assert (it, th) == (it0, th0)
         assert np.array_equal(b.cpu().numpy(), W)
+
+
+def test_landau_restart_policy():
+    """glu_gfix_landau against the policy of Gfix/Landau.c:301-345 replayed
+    by hand: attempt 0 on the input, attempt k on the input after the
+    random transform seeded restart_seed + k; the first success ends it and
+    the count is 123456789 + (k - 1) max_iters + its iterations."""
+    dims = (8, 8, 8, 8)
+    U, _ = heatbath_config(dims, sweeps=20)
+    seed = 5
+    found = False
+    with gl.Gfix(dims, gl.LANDAU, gl.FP64) as g:
+        g.set_links(U)
+        g.snapshot()
+        full, _ = g.run(gl.CG, 0.08, 1e-14, 1000)
+        for M in range(full // 3, full, max(1, full // 12)):
+            g.restore()
+            it0, _ = g.run(gl.CG, 0.08, 1e-14, M)
+            if it0 != 123456789:
+                continue
+            outcome = None  # (k, loc, theta) of the first successful restart
+            for k in range(1, 7):
+                g.restore()
+                g.random_transform(seed + k)
+                loc, th = g.run(gl.CG, 0.08, 1e-14, M)
+                if loc != 123456789:
+                    outcome = (k, loc, th, g.get_links())
+                    break
+            g.restore()
+            if outcome is None:
+                with pytest.raises(gl.GfixError):
+                    g.landau(gl.CG, 0.08, 1e-14, M, restart_seed=seed)
+                continue
+            k, loc, th, W = outcome
+            it, th2, nf = g.landau(gl.CG, 0.08, 1e-14, M, restart_seed=seed)
+            assert nf == k and it == 123456789 + (k - 1) * M + loc and th2 == th
+            assert np.array_equal(g.get_links(), W)
+            found = True
+            break
+    assert found, "no max_iters gave a failure followed by a successful restart"
+
+
+def test_server_reports_failure_and_keeps_serving():
+    dims = (8, 8, 8, 8)
+    U, _ = heatbath_config(dims, sweeps=20)
+    W, it0, th0 = alone(dims, U, precision=gl.MIXED, landau=True)
+    bad = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+    good = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+    torch.cuda.synchronize()
+    with gl.Server(max_active=2) as srv:
+        t_bad = srv.submit(dims, bad.data_ptr(), precision=gl.MIXED, max_iters=5)
+        t_good = srv.submit(dims, good.data_ptr(), precision=gl.MIXED)
+        with pytest.raises(gl.GfixError):
+            srv.wait(t_bad)
+        it, th, _ = srv.wait(t_good)
+    assert (it, th) == (it0, th0)
+    assert np.array_equal(good.cpu().numpy(), W)
