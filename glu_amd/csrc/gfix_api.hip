@@ -108,6 +108,14 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
   for (int mu = 1; mu < 4; mu++) c->g.S[mu] = c->g.S[mu - 1] * (long long)dims[mu - 1];
   c->g.V = c->g.S[3] * (long long)dims[3];
   c->g.Lcu = c->g.S[3];
+  if (c->g.V >= (1ll << 31)) {
+    const long long vol = c->g.V;
+    delete c;
+    return gfix_fail(-1, "lattice volume %lld is beyond the supported 2^31 - 1 sites", vol);
+  }
+  geo_set_div(c->g);
+  c->g.xcd = 1;
+  if (const char *e = getenv("GLU_GFIX_XCD")) c->g.xcd = atoi(e) ? 1 : 0;
   c->gftype = gftype;
   c->precision = precision;
   const long long V = c->g.V;

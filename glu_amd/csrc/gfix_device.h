@@ -18,18 +18,55 @@ constexpr int kWave = 64;
 struct Geo {
   int L[4];
   long long S[4];  // strides 1, Lx, LxLy, LxLyLz
-  long long V;
+  long long V;     // < 2^31 (checked at context creation)
   long long Lcu;
+  // n / L[mu] = umulhi(n, dm[mu]) >> ds[mu] for 0 <= n < 2^31
+  // (round-up multiplier of Granlund and Montgomery, "Division by
+  // invariant integers using multiplication", 1994, Thm 4.2 with N = 31)
+  unsigned dm[4];
+  int ds[4];
+  int xcd;  // 1: XCD-contiguous block order in the stencil kernels
 };
 
+inline void geo_set_div(Geo &g) {
+  for (int mu = 0; mu < 4; mu++) {
+    const unsigned long long d = (unsigned long long)g.L[mu];
+    int l = 0;
+    while ((1ull << l) < d) l++;  // ceil(log2 d), d >= 2
+    g.dm[mu] = (unsigned)(((1ull << (31 + l)) + d - 1) / d);
+    g.ds[mu] = l - 1;
+  }
+}
+
+__device__ __forceinline__ unsigned geo_div(const Geo &g, int mu, unsigned n) {
+  return __umulhi(n, g.dm[mu]) >> g.ds[mu];
+}
+
+// Logical block of this workgroup.  A dispatch hands workgroups to the 8
+// XCDs round-robin, and each XCD has its own L2.  With g.xcd set, XCD k
+// gets a contiguous run of logical blocks, so the neighbour sites a
+// stencil reads (+-x, +-y in the block, +-z a few blocks away) were
+// fetched into that XCD's L2 by a block just before.  Partial sums are
+// stored by logical block, so the reduction order -- and every result --
+// does not depend on the mapping.
+__device__ __forceinline__ unsigned logical_block(const Geo &g) {
+  const unsigned b = blockIdx.x;
+  if (!g.xcd) return b;
+  const unsigned nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
+  return x * q + (x < r ? x : r) + k;
+}
+
 __device__ __forceinline__ void coords(const Geo &g, long long i, int x[4]) {
-  long long r = i;
-  x[0] = (int)(r % g.L[0]);
-  r /= g.L[0];
-  x[1] = (int)(r % g.L[1]);
-  r /= g.L[1];
-  x[2] = (int)(r % g.L[2]);
-  x[3] = (int)(r / g.L[2]);
+  unsigned r = (unsigned)i;
+  unsigned q = geo_div(g, 0, r);
+  x[0] = (int)(r - q * (unsigned)g.L[0]);
+  r = q;
+  q = geo_div(g, 1, r);
+  x[1] = (int)(r - q * (unsigned)g.L[1]);
+  r = q;
+  q = geo_div(g, 2, r);
+  x[2] = (int)(r - q * (unsigned)g.L[2]);
+  x[3] = (int)q;
 }
 __device__ __forceinline__ long long nb_bwd(const Geo &g, long long i, const int x[4], int mu) {
   return x[mu] > 0 ? i - g.S[mu] : i + (long long)(g.L[mu] - 1) * g.S[mu];

@@ -18,7 +18,8 @@ __global__ __launch_bounds__(kBlock) void k_deriv(const vec2<T> *__restrict__ U,
                                                  double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
   if (ctrl && ctrl->done) return;
   const long long V = g.V;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned lb = logical_block(g);
+  const long long i = (long long)lb * blockDim.x + threadIdx.x;
   double loc[1] = {0.0};
   if (i < V) {
     int x[4];
@@ -48,7 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_deriv(const vec2<T> *__restrict__ U,
              ((double)R1 * R1 + (double)I1 * I1 + (double)R2 * R2 + (double)I2 * I2 + (double)R4 * R4 +
               (double)I4 * I4);
   }
-  block_sum_store<1>(loc, part + blockIdx.x);
+  block_sum_store<1>(loc, part + lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -145,7 +146,8 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
   if (ctrl->done) return;
   if (!force && !(ctrl->theta > ctrl->cg_tol)) return;
   const long long V = g.V;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned lb = logical_block(g);
+  const long long i = (long long)lb * blockDim.x + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (i < V) {
     int x[4];
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
       v[pr] = acc;
     }
   }
-  block_sum_store<3>(v, part + 3 * blockIdx.x);
+  block_sum_store<3>(v, part + 3 * lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -182,7 +184,7 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
                                                   int fixed_step) {
   if (ctrl->done) return;
   const long long V = g.V;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long i = (long long)logical_block(g) * blockDim.x + threadIdx.x;
   if (i >= V) return;
   const double alpha = fixed_step ? ctrl->gf_alpha : ctrl->alpha;
   int x[4];
