@@ -35,6 +35,8 @@ PEAK_HBM_GBS = 8000.0
 ALG_BYTES = {
     "deriv": (640, 320), "fft_axis": (128, 64), "pr_sums": (128, 64),
     "cg_update": (256, 128), "line_search": (640, 320), "egauge": (1216, 608),
+    # the gauge update fused with the next iteration's derivative
+    "egauge_deriv": (1216 + 640, 608 + 320),
 }
 # per Landau FACG iteration (FP64 / FP32): the sum over the logical steps
 ALG_ITER = (3136, 1568)
@@ -43,7 +45,7 @@ ALG_ITER = (3136, 1568)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dims", default="32,32,32,64")
     ap.add_argument("--sweeps", type=int, default=200)
@@ -73,36 +75,48 @@ def kernel_mask(name, prec):
     return 1 << (gl.KERNELS.index(name) + (len(gl.KERNELS) if prec == "fp32" else 0))
 
 
-def timed_runs(g, steps, warmup):
-    """Warm-up runs (the first one times every kernel class, to find the
+def solve(g, seed):
+    """One step: the input config under its own seeded random gauge
+    transform, gauge fixed with Landau()'s semantics (restarts included)."""
+    g.restore()
+    g.random_transform(seed=seed)
+    it, th, restarts = g.landau(0, 0.08, 1e-14, 1000, restart_seed=seed)
+    if not th < 1e-14:
+        raise RuntimeError(f"did not converge: theta {th}")
+    return it, restarts
+
+
+def timed_runs(g, steps, warmup, seed):
+    """Warm-up steps (the first one times every kernel class, to find the
     dominant kernel), then the timed steps with HIP events around the
-    dominant kernel's launches only."""
+    dominant kernel's launches only.  Step k fixes gauge copy seed + k of
+    the input, so every precision sees the same copies."""
     prof_all = None
     for w in range(max(warmup, 1)):
-        g.restore()
         if w == 0:
             g.profile(-1)
-        g.run(0, 0.08, 1e-14, 1000)
+        solve(g, seed + 1000 + w)
         if w == 0:
             prof_all = g.profile_read()
             g.profile(0)
     dom = max(prof_all.items(), key=lambda kv: kv[1][2])[0]
     g.measure()  # synchronises the device
-    lat, its = [], []
+    lat, its, restarts, split, func = [], [], 0, [0, 0], []
     g.profile(kernel_mask(*dom))
-    t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         t = time.perf_counter()
-        g.restore()
-        it, th = g.run(0, 0.08, 1e-14, 1000)
+        it, nr = solve(g, seed + k)
         lat.append(time.perf_counter() - t)
         its.append(it)
-        if not th < 1e-14:
-            raise RuntimeError(f"did not converge: theta {th}")
-    total = time.perf_counter() - t0
+        restarts += nr
+        n32, n64 = g.mixed_split()
+        split[0] += n32
+        split[1] += n64
+        func.append(g.measure()["functional"])  # outside the step's clock
+    total = sum(lat)
     live = g.profile_read()
     g.profile(0)
-    return total, lat, its, prof_all, dom, live[dom]
+    return total, lat, its, restarts, split, func, prof_all, dom, live[dom]
 
 
 def main():
@@ -127,28 +141,35 @@ def main():
     t = time.perf_counter()
     gm.heatbath(6.0, a.sweeps, 4, seed=a.seed, cold_start=True)
     plaq = gm.measure()["plaquette"]
-    gm.random_transform(seed=a.seed + 1)
     gm.snapshot()
     print(f"[bench] input ready in {time.perf_counter() - t:.1f}s, plaquette {plaq:.12f}",
           file=sys.stderr, flush=True)
+    tune = gm.autotune()  # launch shapes, before any timing
+    print("[bench] autotune:\n" + tune.rstrip(), file=sys.stderr, flush=True)
 
-    total, lat, its, prof, dom, dom_live = timed_runs(gm, a.steps, a.warmup)
-    n32, n64 = gm.mixed_split()
-    gm.restore()
+    total, lat, its, restarts, (n32, n64), func, prof, dom, dom_live = timed_runs(
+        gm, a.steps, a.warmup, a.seed + 1)
     mixed_value = a.steps / total
-    print(f"[bench] mixed: {mixed_value:.3f} configs/s, iters {its} (fp32 {n32} + fp64 {n64})",
-          file=sys.stderr, flush=True)
+    print(f"[bench] mixed: {mixed_value:.3f} configs/s, iters {its} (fp32 {n32} + fp64 {n64} in all), "
+          f"restarts {restarts}", file=sys.stderr, flush=True)
 
     fp64 = None
     if not a.no_fp64:
+        gm.restore()
         U = gm.get_links()
         g64 = gl.Gfix(dims, gl.LANDAU, gl.FP64)
         g64.set_links(U)
         g64.snapshot()
-        t64, lat64, its64, prof64, _, _ = timed_runs(g64, a.steps, a.warmup)
+        g64.autotune()
+        t64, lat64, its64, rs64, _, func64, prof64, _, _ = timed_runs(g64, a.steps, a.warmup, a.seed + 1)
+        rel = [abs(x - y) / abs(y) for x, y in zip(func, func64)]
         fp64 = {"configs_per_s": a.steps / t64, "ms_per_step": 1e3 * t64 / a.steps,
-                "iters": its64, "p50_ms": 1e3 * pct(lat64, 0.5)}
-        print(f"[bench] fp64: {fp64['configs_per_s']:.3f} configs/s, iters {its64}",
+                "iters": its64, "restarts": rs64, "p50_ms": 1e3 * pct(lat64, 0.5),
+                # error of the mixed result against the FP64 one, per gauge copy:
+                # ~1e-13 on the same Gribov copy, 1e-6..1e-4 on a neighbouring one
+                "functional_rel_diff_mixed": [float(f"{r:.2e}") for r in rel]}
+        print(f"[bench] fp64: {fp64['configs_per_s']:.3f} configs/s, iters {its64}, restarts {rs64}; "
+              f"functional rel. diff mixed vs fp64 {fp64['functional_rel_diff_mixed']}",
               file=sys.stderr, flush=True)
         prof.update({k: v for k, v in prof64.items() if k not in prof})
 
@@ -169,25 +190,30 @@ def main():
                              "GBps": round(ALG_BYTES[k][0 if p == "fp64" else 1] * V / (m * 1e-3) / 1e9, 1)}
                for (k, p), (n, m, tt) in sorted(prof.items())}
 
-    # streaming rate of the whole mixed solve from the algorithmic bytes
+    # streaming rate of the mixed solves from the algorithmic bytes
     alg_total = (n32 * ALG_ITER[1] + n64 * ALG_ITER[0]) * V
-    gbs = alg_total / (sum(lat) / len(lat)) / 1e9
+    gbs = alg_total / total / 1e9
 
     cpu = None
     if not a.no_cpu:
-        cpu = cpu_baseline(gm, dims, fp64["iters"][0] if fp64 else its[0], a.cpu_iters)
+        ref_its = [i for i in (fp64["iters"] if fp64 else its) if i < 123456789]
+        gm.restore()
+        gm.random_transform(seed=a.seed + 1)
+        cpu = cpu_baseline(gm, dims, sorted(ref_its)[len(ref_its) // 2], a.cpu_iters)
 
     line = {
         "metric": METRIC, "value": round(mixed_value, 4), "unit": "configs/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, "
-                 f"4 overrelaxation each, seed {a.seed}) + seeded random gauge transform"),
+                 f"4 overrelaxation each, seed {a.seed}); step k fixes its own gauge copy "
+                 f"(random gauge transform seed {a.seed + 1}+k), with Landau() restarts"),
         "config": {"workload": "landau_facg_mixed_" + "x".join(map(str, dims)),
                    "lattice": list(dims), "gf_tune": 0.08, "accuracy": 1e-14, "max_iters": 1000,
                    "switch_theta": a.switch_theta},
         "p50_ms": round(1e3 * pct(lat, 0.5), 2), "p99_ms": round(1e3 * pct(lat, 0.99), 2),
-        "iters": its, "iters_fp32": n32, "iters_fp64": n64, "stream_GBps": round(gbs, 1),
+        "iters": its, "restarts": restarts, "iters_fp32": n32, "iters_fp64": n64,
+        "stream_GBps": round(gbs, 1),
         "fp64": fp64,
         "mixed_speedup_vs_fp64": round(mixed_value / fp64["configs_per_s"], 3) if fp64 else None,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
@@ -196,13 +222,13 @@ def main():
 
 
 def cpu_baseline(g, dims, iters_fp64, k):
-    """The CPU oracle (C, OpenMP) on the same input: time k and 2k FACG
-    iterations, take the per-iteration cost from the difference and scale
-    it by the FP64 GPU iteration count of the full solve."""
+    """The CPU oracle (C, OpenMP) on the context's current links (the first
+    timed gauge copy): time k and 2k FACG iterations, take the
+    per-iteration cost from the difference and scale it by the median FP64
+    GPU iteration count of a full solve."""
     from oracle import oracle as O
 
     nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    g.restore()
     U = g.get_links()
     L = O.Lattice(dims, nth=nth)
     ts = []
@@ -221,7 +247,10 @@ def cpu_baseline(g, dims, iters_fp64, k):
                        "the full FP64 solve")}
 
 
-COULOMB_ALG = (576 + 144 + 64) + 4 * 128 + 128 + 256 + (576 + 144 + 64) + (144 + 64 + 144)
+# per slice-site and slice iteration (FP64): derivative on rotated links,
+# forward + backward FFT credited once each, PR sums, CG update, line
+# search, slice-transform update
+COULOMB_ALG = (576 + 144 + 64) + 2 * 128 + 128 + 256 + (576 + 144 + 64) + (144 + 64 + 144)
 
 
 def coulomb_main(a):

@@ -32,12 +32,12 @@ EXPORTS = [
     "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
-    "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau",
+    "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy",
 ]
 
-KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge"]
+KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv"]
 
 
 class GfixError(RuntimeError):
@@ -80,6 +80,7 @@ def lib():
             "glu_gfix_profile_read": (C.c_int, [P, C.c_int, C.POINTER(SZ), DP, DP]),
             "glu_gfix_get_links_device": (C.c_int, [P, C.c_void_p]),
             "glu_gfix_set_graph": (C.c_int, [P, C.c_int]),
+            "glu_gfix_autotune": (C.c_int, [P, C.c_char_p, SZ]),
             "glu_gfix_server_create": (C.c_int, [C.POINTER(P), SZ, C.c_int]),
             "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int, C.c_uint64]),
             "glu_gfix_landau": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, C.c_uint64, DP,
@@ -145,6 +146,12 @@ class Gfix:
 
     def get_links_device(self, ptr: int):
         _check(lib().glu_gfix_get_links_device(self._h, C.c_void_p(ptr)), "get_links_device")
+
+    def autotune(self) -> str:
+        """Pick the fastest launch shapes for this context; returns the timings."""
+        buf = C.create_string_buffer(4096)
+        _check(lib().glu_gfix_autotune(self._h, buf, len(buf)), "autotune")
+        return buf.value.decode()
 
     def set_graph(self, on: bool):
         _check(lib().glu_gfix_set_graph(self._h, int(bool(on))), "set_graph")
@@ -218,7 +225,7 @@ class Gfix:
         _check(lib().glu_gfix_random_transform(self._h, seed), "random_transform")
 
     def profile(self, mask=-1):
-        """Time kernel classes with HIP events (bit k = KERNELS[k % 6], +6
+        """Time kernel classes with HIP events (bit k = KERNELS[k % 7], +7
         for FP32); -1 all, 0 off."""
         _check(lib().glu_gfix_profile(self._h, int(mask)), "profile")
 

@@ -336,12 +336,12 @@ template bool launch_fft_xy<double>(void *, long long, int, const FftAxis &, con
 template bool launch_fft_xy<float>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
                                    hipStream_t, long long, int, long long, const int *);
 
-template <typename T, int N>
+template <typename T, int N, int P>
 __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ F, long long fstride, long long S,
                                                          long long npencil, const double2 *__restrict__ tw, int mode,
                                                          PsqInfo ps, const Ctrl *__restrict__ ctrl,
                                                          long long slice_len, const int *__restrict__ slice_done) {
-  constexpr int P = fast_P(N), LD = P + 1, TOT = N * P;
+  constexpr int LD = P + 1, TOT = N * P;
   static_assert(LD > 0, "");
   static_assert(kFftThreads % P == 0, "a pencil per thread needs P | threads");
   if (ctrl && ctrl->done) return;
@@ -406,16 +406,35 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ 
   }
 }
 
+// pencils per workgroup of variant v: the default fast_P(N), half of it,
+// or twice it where the LDS tile stays within 64 KB
+template <typename T, int N> constexpr int variant_P(int v) {
+  constexpr int P0 = fast_P(N);
+  if (v == 1) return P0 > 1 ? P0 / 2 : P0;
+  if (v == 2) return (2 * P0 <= kFftThreads && N * (2 * P0 + 1) * (int)sizeof(vec2<T>) <= 65536) ? 2 * P0 : P0;
+  return P0;
+}
+
+template <typename T, int N, int P>
+static void run_fast(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+  const unsigned nb = (unsigned)((ax.npencil + P - 1) / P);
+  hipLaunchKernelGGL((k_fft_fast<T, N, P>), dim3(nb, nfield, nslice), dim3(kFftThreads), 0, s, (vec2<T> *)F,
+                     fstride, ax.S, ax.npencil, ax.tw, mode, ps, ctrl, slice_len, slice_done);
+}
+
 template <typename T, int N>
 static bool try_fast(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
                      const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
   if (ax.N != N) return false;
-  constexpr int P = fast_P(N);
   // the fused p^2 pass runs on a strided axis (it is always the last one)
   if (mode == 2 && ax.S == 1) return false;
-  const unsigned nb = (unsigned)((ax.npencil + P - 1) / P);
-  hipLaunchKernelGGL((k_fft_fast<T, N>), dim3(nb, nfield, nslice), dim3(kFftThreads), 0, s, (vec2<T> *)F, fstride,
-                     ax.S, ax.npencil, ax.tw, mode, ps, ctrl, slice_len, slice_done);
+  if (ax.pv == 1)
+    run_fast<T, N, variant_P<T, N>(1)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
+  else if (ax.pv == 2)
+    run_fast<T, N, variant_P<T, N>(2)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
+  else
+    run_fast<T, N, variant_P<T, N>(0)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
   return true;
 }
 

@@ -70,10 +70,10 @@ size_t pow2_floor(size_t x) {
 
 size_t gfix_footprint(const size_t dims[4], int gftype, int precision) {
   const size_t V = dims[0] * dims[1] * dims[2] * dims[3];
-  size_t b = V * 4 * 6 * sizeof(double2);     // FP64 links
+  size_t b = 2 * V * 4 * 6 * sizeof(double2);  // FP64 links, current and next
   b += 3 * V * 4 * sizeof(double2);           // in, sn, in_old
   if (precision == GLU_GFIX_MIXED)
-    b += V * 4 * 6 * sizeof(float2) + V * 6 * sizeof(double2) + 3 * V * 4 * sizeof(float2);
+    b += 2 * V * 4 * 6 * sizeof(float2) + V * 6 * sizeof(double2) + 3 * V * 4 * sizeof(float2);
   if (gftype == GLU_GFIX_COULOMB) b += 2 * V * 6 * sizeof(double2);  // slice transforms + scratch
   b += ((V + kBlock - 1) / kBlock) * 4 * sizeof(double) + (64 << 10);
   return b;
@@ -135,6 +135,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     CK(hipEventCreateWithFlags(&c->run_ev, hipEventDisableTiming));
     if (const char *e = getenv("GLU_GFIX_GRAPH")) c->use_graph = atoi(e);
     CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
+    CK(hipMalloc(&c->U64b, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
     CK(hipMalloc(&c->sn, (size_t)V * 4 * sizeof(double2)));
     CK(hipMalloc(&c->old, (size_t)V * 4 * sizeof(double2)));
@@ -144,6 +145,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     CK(hipMalloc(&c->trace, (size_t)c->trace_cap * 3 * sizeof(double)));
     if (precision == GLU_GFIX_MIXED) {
       CK(hipMalloc(&c->U32, (size_t)V * 24 * sizeof(float2)));
+      CK(hipMalloc(&c->U32b, (size_t)V * 24 * sizeof(float2)));
       CK(hipMalloc(&c->G64, (size_t)V * 6 * sizeof(double2)));
       CK(hipMalloc(&c->F32, (size_t)V * 4 * sizeof(float2)));
       CK(hipMalloc(&c->sn32, (size_t)V * 4 * sizeof(float2)));
@@ -201,7 +203,7 @@ bad:
 extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *bufs[] = {c->U64, c->U32, c->G64, c->Usnap, c->Uorig, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
+  void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
                   c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -363,6 +365,25 @@ template <typename T> static Bufs bufs_for(glu_gfix_ctx *c) {
 }
 
 // steepest-descent start of a FACG segment (Gfix/FACG.c:249-265)
+template <typename T> static constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
+
+// the CG gauge update, out of place and fused with the next derivative;
+// the host's link pointer moves to the buffer it writes
+template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
+  void *&cur = sizeof(T) == 4 ? c->U32 : c->U64;
+  void *&nxt = sizeof(T) == 4 ? c->U32b : c->U64b;
+  const Bufs b = bufs_for<T>(c);
+  GFIX_PROF(c, pk<T>(PK_EGDERIV),
+            launch_egauge_deriv<T>(cur, nxt, b.sn, b.G, b.F, c->part, c->g, c->ctrl, c->stream));
+  std::swap(cur, nxt);
+  c->upar[tix<T>()] ^= 1;
+  c->deriv_ready[tix<T>()] = 1;
+}
+
+// SD start of steep_Landau_FACG, Gfix/FACG.c:249-265.  sn = in_old = in and
+// inold = <in, in> depend on `in` only, so they are taken before the update
+// (which overwrites `in` with the next derivative) and the update reads the
+// direction from the copy in sn.
 template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
   launch_begin_segment(c->ctrl, 1, s);
@@ -371,16 +392,19 @@ template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   launch_line_search<T>(b.U, b.F, c->part, c->g, c->ctrl, 1, s);
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 1, 0.15, 0.30, s);
-  launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 0, s);
   launch_copy3<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s);
   launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
   launch_fin_sd(c->part, c->nblk, c->ctrl, s);
+  enq_update_deriv<T>(c);
 }
 
 // one FACG iteration (Gfix/FACG.c:272-334)
 template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
-  GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
+  if (!c->deriv_ready[tix<T>()]) {
+    GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
+    c->deriv_ready[tix<T>()] = 1;
+  }
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
@@ -388,7 +412,7 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s));
   GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, c->g, c->ctrl, 0, s));
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
-  GFIX_PROF(c, pk<T>(PK_EGAUGE), launch_egauge<T>(b.U, b.sn, b.G, c->g, c->ctrl, 0, s));
+  enq_update_deriv<T>(c);
   launch_iter_end(c->ctrl, c->trace, 0, s);
 }
 
@@ -418,6 +442,7 @@ static int write_ctrl(glu_gfix_ctx *c) {
 
 // theta_test_lin = 2 * (FA theta) on the links U (Gfix/gftests.c:215-250)
 template <typename T> static int theta_lin(glu_gfix_ctx *c, void *U, double *out) {
+  c->deriv_ready[0] = c->deriv_ready[1] = 0;  // F and the partials are reused here
   launch_deriv<T>(U, c->F, c->part, c->g, nullptr, c->stream);
   std::vector<double> h(c->nblk);
   if (hipMemcpyAsync(h.data(), c->part, c->nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -468,12 +493,11 @@ static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, siz
 }
 
 template <typename T> static void enq_iterations(glu_gfix_ctx *c) {
-  const Bufs b = bufs_for<T>(c);
   for (long long k = 0; k < kBatch; k++) {
     if (c->run.method == GLU_GFIX_CG)
-      enq_cg_iter<T>(c, b);
+      enq_cg_iter<T>(c, bufs_for<T>(c));
     else
-      enq_sd_iter<T>(c, b);
+      enq_sd_iter<T>(c, bufs_for<T>(c));
   }
 }
 
@@ -483,7 +507,15 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
     enq_iterations<T>(c);
     return 0;
   }
-  const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1);
+  // a pending derivative is enqueued outside the graph, so every captured
+  // batch starts and ends with it in place (and, 8 swaps later, on the
+  // same link buffer it started on)
+  if (c->run.method == GLU_GFIX_CG && !c->deriv_ready[tix<T>()]) {
+    const Bufs b = bufs_for<T>(c);
+    launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, c->stream);
+    c->deriv_ready[tix<T>()] = 1;
+  }
+  const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1) + 4 * c->upar[tix<T>()];
   if (!c->graph[kind]) {
     hipGraph_t gr = nullptr;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
@@ -528,6 +560,7 @@ int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_
   r.max_iters = max_iters;
   c->trace_n = 0;
   c->last_fp32_iters = c->last_fp64_iters = 0;
+  c->deriv_ready[0] = c->deriv_ready[1] = 0;
   double th0 = 0.0;
   if (int rc = theta_lin<double>(c, c->U64, &th0)) return rc;
   if (th0 < acc) {  // Gfix/FACG.c:349-353

@@ -10,7 +10,7 @@
 struct CoulombState;
 
 // per-kernel event timing (glu_gfix_profile): kernel classes
-enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_COUNT };
+enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_EGDERIV, PK_COUNT };
 struct ProfPending {
   int id;
   hipEvent_t a, b;
@@ -40,12 +40,20 @@ struct glu_gfix_ctx {
   // FP32 accumulated site transform for the mixed schedule, snapshot
   void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
   void *Uorig = nullptr;  // input links kept for the Landau() restarts
+  // second link buffers: the CG gauge update writes out of place and the
+  // host swaps U64 <-> U64b (U32 <-> U32b) as it enqueues it; upar counts
+  // the swaps mod 2 (FP64, FP32) so captured graphs know their start
+  void *U64b = nullptr, *U32b = nullptr;
+  int upar[2] = {0, 0};
+  // the fused update left the derivative of the current links (fields and
+  // theta partials) in place: the next CG iteration needs no k_deriv
+  int deriv_ready[2] = {0, 0};
   // FP32 packed fields of the mixed schedule (in, sn, in_old)
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
   double switch_theta = 1e-7;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
   int use_graph = 0;  // replay iteration batches from captured HIP graphs
-  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};  // CG/SD x FP64/FP32
+  hipGraphExec_t graph[8] = {};  // CG/SD x FP64/FP32 x link-buffer parity
   hipEvent_t run_ev = nullptr;
   RunState run;
   // packed fields (4 x V complex each): in, sn (search direction), in_old

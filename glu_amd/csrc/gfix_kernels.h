@@ -18,6 +18,7 @@ struct FftAxis {
   int nst;
   int rad[8];
   const double2 *tw;  // N entries exp(-2 pi i k / N)
+  int pv;             // pencils-per-workgroup variant of the LDS kernel (tuned)
 };
 
 // closed-form p^2 preconditioner of the fused pass
@@ -56,6 +57,9 @@ void launch_line_search(const void *U, const void *F, double *part, const Geo &g
                         hipStream_t s);
 template <typename T>
 void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s);
+template <typename T>
+void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fout, double *part, const Geo &g,
+                         const Ctrl *ctrl, hipStream_t s);
 template <typename T>
 void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s);
 template <typename T>

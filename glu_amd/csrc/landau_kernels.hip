@@ -13,6 +13,36 @@ namespace gfix {
 // ---------------------------------------------------------------------------
 // derivative: Gfix/lin_derivs.c:75-193 + packing Gfix/FACG.c:38-73
 // ---------------------------------------------------------------------------
+// Delta(x) = sum_mu [A_mu(x - mu) - A_mu(x)] from the own link q = U_mu(x)
+// and the backward link s = U_mu(x - mu), accumulated direction by
+// direction, then packed into the 4 fields with the local tr Delta^2
+template <typename T> struct DerivAcc {
+  T R0 = 0, R1 = 0, I1 = 0, R2 = 0, I2 = 0, R3 = 0, R4 = 0, I4 = 0;
+  __device__ __forceinline__ void add(const su3<T> &q, const su3<T> &s) {
+    R0 += T(2) * s.e[0].im - s.e[4].im - s.e[8].im;
+    R0 += T(-2) * q.e[0].im + q.e[4].im + q.e[8].im;
+    R1 += s.e[1].re - s.e[3].re - q.e[1].re + q.e[3].re;
+    I1 += s.e[1].im + s.e[3].im - q.e[1].im - q.e[3].im;
+    R2 += s.e[2].re - s.e[6].re - q.e[2].re + q.e[6].re;
+    I2 += s.e[2].im + s.e[6].im - q.e[2].im - q.e[6].im;
+    R3 += T(2) * s.e[4].im - s.e[0].im - s.e[8].im;
+    R3 += T(-2) * q.e[4].im + q.e[0].im + q.e[8].im;
+    R4 += s.e[5].re - s.e[7].re - q.e[5].re + q.e[7].re;
+    I4 += s.e[5].im + s.e[7].im - q.e[5].im - q.e[7].im;
+  }
+  // packing of Gfix/FACG.c:57-60; returns tr Delta^2 (lin_derivs.c:180-183)
+  __device__ __forceinline__ double store(vec2<T> *__restrict__ F, long long V, long long i) const {
+    st<T>(F + i, mk<T>(-R3 / T(3), R0 / T(3)));
+    st<T>(F + V + i, mk<T>(T(0.5) * R1, T(0.5) * I1));
+    st<T>(F + 2 * V + i, mk<T>(T(0.5) * R2, T(0.5) * I2));
+    st<T>(F + 3 * V + i, mk<T>(T(0.5) * R4, T(0.5) * I4));
+    const double r0 = R0, r3 = R3;
+    return 0.4444444444444444 * (r0 * r0 + r3 * (r3 + r0)) +
+           ((double)R1 * R1 + (double)I1 * I1 + (double)R2 * R2 + (double)I2 * I2 + (double)R4 * R4 +
+            (double)I4 * I4);
+  }
+};
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_deriv(const vec2<T> *__restrict__ U, vec2<T> *__restrict__ F,
                                                  double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
@@ -24,30 +54,11 @@ __global__ __launch_bounds__(kBlock) void k_deriv(const vec2<T> *__restrict__ U,
   if (i < V) {
     int x[4];
     coords(g, i, x);
-    T R0 = 0, R1 = 0, I1 = 0, R2 = 0, I2 = 0, R3 = 0, R4 = 0, I4 = 0;
+    DerivAcc<T> d;
 #pragma unroll
-    for (int mu = 0; mu < kND; mu++) {
-      const su3<T> q = load_link<T>(U, mu, i, V);
-      const su3<T> s = load_link<T>(U, mu, nb_bwd(g, i, x, mu), V);
-      R0 += T(2) * s.e[0].im - s.e[4].im - s.e[8].im;
-      R0 += T(-2) * q.e[0].im + q.e[4].im + q.e[8].im;
-      R1 += s.e[1].re - s.e[3].re - q.e[1].re + q.e[3].re;
-      I1 += s.e[1].im + s.e[3].im - q.e[1].im - q.e[3].im;
-      R2 += s.e[2].re - s.e[6].re - q.e[2].re + q.e[6].re;
-      I2 += s.e[2].im + s.e[6].im - q.e[2].im - q.e[6].im;
-      R3 += T(2) * s.e[4].im - s.e[0].im - s.e[8].im;
-      R3 += T(-2) * q.e[4].im + q.e[0].im + q.e[8].im;
-      R4 += s.e[5].re - s.e[7].re - q.e[5].re + q.e[7].re;
-      I4 += s.e[5].im + s.e[7].im - q.e[5].im - q.e[7].im;
-    }
-    st<T>(F + i, mk<T>(-R3 / T(3), R0 / T(3)));
-    st<T>(F + V + i, mk<T>(T(0.5) * R1, T(0.5) * I1));
-    st<T>(F + 2 * V + i, mk<T>(T(0.5) * R2, T(0.5) * I2));
-    st<T>(F + 3 * V + i, mk<T>(T(0.5) * R4, T(0.5) * I4));
-    const double r0 = R0, r3 = R3;
-    loc[0] = 0.4444444444444444 * (r0 * r0 + r3 * (r3 + r0)) +
-             ((double)R1 * R1 + (double)I1 * I1 + (double)R2 * R2 + (double)I2 * I2 + (double)R4 * R4 +
-              (double)I4 * I4);
+    for (int mu = 0; mu < kND; mu++)
+      d.add(load_link<T>(U, mu, i, V), load_link<T>(U, mu, nb_bwd(g, i, x, mu), V));
+    loc[0] = d.store(F, V, i);
   }
   block_sum_store<1>(loc, part + lb);
 }
@@ -204,6 +215,60 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
     const su3<double> A64 = gauge_at64<T>(F, V, i, alpha);
     store_link<double>(G, 0, i, V, mul_cols<double>(A64, load_link<double>(G, 0, i, V)));
   }
+}
+
+// ---------------------------------------------------------------------------
+// The gauge update of one CG iteration fused with the derivative of the
+// next (Gfix/FACG.c:325-334 followed by :276-280).  The update is out of
+// place, Uin -> Uout: each site also forms its backward links
+// U'_mu(x - mu) = g(x - mu) U_mu(x - mu) g(x)^dag from the old links, with
+// the arithmetic of the site that stores them, and the old links of the
+// neighbours must stay readable meanwhile.  The derivative of the updated
+// links then costs no second pass over them.  D is the search direction;
+// Fout receives the packed derivative and part its theta partials.  Past
+// convergence the links are copied unchanged, so the buffer swap the host
+// made at enqueue time holds whichever iteration stopped.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_egauge_deriv(const vec2<T> *__restrict__ Uin, vec2<T> *__restrict__ Uout,
+                                                        const vec2<T> *__restrict__ D, double2 *__restrict__ G,
+                                                        vec2<T> *__restrict__ Fout, double *__restrict__ part, Geo g,
+                                                        const Ctrl *__restrict__ ctrl) {
+  const long long V = g.V;
+  const unsigned lb = logical_block(g);
+  const long long i = (long long)lb * blockDim.x + threadIdx.x;
+  if (ctrl->done) {
+    if (i < V)
+#pragma unroll 4
+      for (int k = 0; k < kND * 6; k++) Uout[(long long)k * V + i] = Uin[(long long)k * V + i];
+    return;
+  }
+  double loc[1] = {0.0};
+  if (i < V) {
+    const double alpha = ctrl->alpha;
+    int x[4];
+    coords(g, i, x);
+    const su3<T> A = gauge_at<T>(D, V, i, (T)alpha);
+    DerivAcc<T> d;
+#pragma unroll 1
+    for (int mu = 0; mu < kND; mu++) {
+      const su3<T> B = gauge_at<T>(D, V, nb_fwd(g, i, x, mu), (T)alpha);
+      su3<T> n = gtransform<T>(A, load_link<T>(Uin, mu, i, V), B);
+      if (sizeof(T) == 4) gram_reunit<T>(n);
+      store_link<T>(Uout, mu, i, V, n);
+      const long long xb = nb_bwd(g, i, x, mu);
+      const su3<T> C = gauge_at<T>(D, V, xb, (T)alpha);
+      su3<T> sb = gtransform<T>(C, load_link<T>(Uin, mu, xb, V), A);
+      if (sizeof(T) == 4) gram_reunit<T>(sb);
+      d.add(n, sb);
+    }
+    loc[0] = d.store(Fout, V, i);
+    if (G) {
+      const su3<double> A64 = gauge_at64<T>(D, V, i, alpha);
+      store_link<double>(G, 0, i, V, mul_cols<double>(A64, load_link<double>(G, 0, i, V)));
+    }
+  }
+  block_sum_store<1>(loc, part + lb);
 }
 
 // ---------------------------------------------------------------------------
@@ -533,6 +598,12 @@ void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ct
                      (double2 *)G, g, ctrl, fixed);
 }
 template <typename T>
+void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fout, double *part, const Geo &g,
+                         const Ctrl *ctrl, hipStream_t s) {
+  hipLaunchKernelGGL(k_egauge_deriv<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)Uin, (vec2<T> *)Uout,
+                     (const vec2<T> *)D, (double2 *)G, (vec2<T> *)Fout, part, g, ctrl);
+}
+template <typename T>
 void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_pack<T>, dim3(nblk(n)), dim3(kBlock), 0, s, aos, (vec2<T> *)U, V, nd, site0, n);
 }
@@ -552,6 +623,8 @@ template <typename T> void launch_identity(void *G, long long V, hipStream_t s) 
   template void launch_line_search<T>(const void *, const void *, double *, const Geo &, const Ctrl *, int,   \
                                       hipStream_t);                                                           \
   template void launch_egauge<T>(void *, const void *, void *, const Geo &, const Ctrl *, int, hipStream_t);  \
+  template void launch_egauge_deriv<T>(const void *, void *, const void *, void *, void *, double *, const Geo &, \
+                                       const Ctrl *, hipStream_t);                                                \
   template void launch_pack<T>(const double *, void *, long long, int, long long, long long, hipStream_t);     \
   template void launch_unpack<T>(const void *, double *, long long, int, long long, long long, hipStream_t);   \
   template void launch_identity<T>(void *, long long, hipStream_t);

@@ -1,0 +1,152 @@
+// tune.hip -- launch-shape autotuner of a context.
+//
+// The tunable shapes are those that change speed and nothing else:
+//   - pencils per workgroup of each LDS FFT axis pass (three variants),
+//   - the fused x-y plane pass against two separate axis passes,
+//   - the XCD-contiguous workgroup order of the stencil kernels.
+// glu_gfix_autotune times every candidate with HIP events on the context's
+// own scratch fields (the FFT passes and the derivative write only the
+// field planes and the partial sums) and keeps the fastest.  The choices
+// are cached per (dims, gftype, precision) for the life of the process, so
+// pooled or repeated contexts of one shape tune once.
+#include <cstdio>
+#include <map>
+#include <mutex>
+#include <string>
+#include <tuple>
+
+#include "../../include/glu_gfix.h"
+#include "gfix_ctx.h"
+
+using namespace gfix;
+
+namespace {
+
+struct Tuning {
+  int pv[4];
+  int use_xy;
+  int xcd;
+  std::string report;
+};
+
+std::mutex g_tune_mu;
+std::map<std::tuple<size_t, size_t, size_t, size_t, int, int>, Tuning> g_tune_cache;
+
+// mean duration of `reps` launches after one untimed launch
+template <class Fn> float time_ms(glu_gfix_ctx *c, Fn &&launch, int reps) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30f;
+  launch();
+  (void)hipEventRecord(a, c->stream);
+  for (int r = 0; r < reps; r++) launch();
+  (void)hipEventRecord(b, c->stream);
+  float ms = 1e30f;
+  if (hipEventSynchronize(b) == hipSuccess && hipEventElapsedTime(&ms, a, b) == hipSuccess) ms /= (float)reps;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
+}
+
+template <typename T> Tuning tune(glu_gfix_ctx *c) {
+  Tuning t{};
+  const int D = c->ps.D;
+  const long long V = c->g.V;
+  const int ns = D == 4 ? 1 : c->g.L[3];
+  const long long sl = D == 4 ? 0 : c->g.Lcu;
+  void *F = sizeof(T) == 4 ? c->F32 : c->F;
+  void *U = sizeof(T) == 4 ? c->U32 : c->U64;
+  constexpr int kReps = 5;
+  char line[160];
+  // FFT axis passes: the last axis is the fused forward * p^2 * backward pass
+  float best_axis[4] = {0, 0, 0, 0};
+  for (int a = 0; a < D; a++) {
+    const int mode = a == D - 1 ? 2 : 0;
+    float best = 1e30f;
+    for (int v = 0; v < 3; v++) {
+      FftAxis ax = c->ax[a];
+      ax.pv = v;
+      const float ms = time_ms(
+          c, [&] { launch_fft_axis<T>(F, V, 4, ax, mode, c->ps, nullptr, c->stream, ns, sl, nullptr); }, kReps);
+      snprintf(line, sizeof(line), "fft axis %d (N=%d) variant %d: %.4f ms\n", a, ax.N, v, ms);
+      t.report += line;
+      if (ms < best) {
+        best = ms;
+        t.pv[a] = v;
+      }
+    }
+    best_axis[a] = best;
+  }
+  // the x-y plane pass against the two axis passes it replaces
+  t.use_xy = 0;
+  const long long nplane = (D == 4 ? V : c->g.Lcu) / c->g.S[2];
+  bool have_xy = false;
+  const float xy = time_ms(
+      c,
+      [&] {
+        have_xy = launch_fft_xy<T>(F, V, 4, c->ax[0], c->ax[1], 0, nullptr, c->stream, nplane, ns, sl, nullptr);
+      },
+      kReps);
+  if (have_xy) {
+    t.use_xy = xy < best_axis[0] + best_axis[1];
+    snprintf(line, sizeof(line), "fft x-y plane pass: %.4f ms against %.4f ms for two passes -> %s\n", xy,
+             best_axis[0] + best_axis[1], t.use_xy ? "plane" : "axes");
+    t.report += line;
+  }
+  // workgroup order of the stencil kernels, timed on the derivative
+  float ord[2];
+  for (int x = 0; x < 2; x++) {
+    Geo g = c->g;
+    g.xcd = x;
+    ord[x] = time_ms(c, [&] { launch_deriv<T>(U, F, c->part, g, nullptr, c->stream); }, kReps);
+  }
+  t.xcd = ord[1] <= ord[0];
+  snprintf(line, sizeof(line), "stencil order: plain %.4f ms, XCD-contiguous %.4f ms -> %s\n", ord[0], ord[1],
+           t.xcd ? "XCD-contiguous" : "plain");
+  t.report += line;
+  return t;
+}
+
+void apply(glu_gfix_ctx *c, const Tuning &t) {
+  for (int a = 0; a < 4; a++) c->ax[a].pv = t.pv[a];
+  c->use_xy = t.use_xy;
+  c->g.xcd = t.xcd;
+  for (hipGraphExec_t &gx : c->graph)  // captured launches used the old shapes
+    if (gx) {
+      (void)hipGraphExecDestroy(gx);
+      gx = nullptr;
+    }
+}
+
+}  // namespace
+
+extern "C" int glu_gfix_autotune(glu_gfix_ctx *c, char *report, size_t cap) {
+  if (!c) return gfix_fail(-1, "null context");
+  if (c->run.inflight) return gfix_fail(-1, "a run is in flight on this context");
+  const auto key = std::make_tuple(c->dims[0], c->dims[1], c->dims[2], c->dims[3], c->gftype, c->precision);
+  Tuning t;
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      t = it->second;
+      cached = true;
+    }
+  }
+  if (!cached) {
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "autotune: stream not idle");
+    // the FP32 kernels carry a mixed Landau run; Coulomb runs in FP64
+    t = (c->precision == GLU_GFIX_MIXED && c->gftype == GLU_GFIX_LANDAU) ? tune<float>(c) : tune<double>(c);
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return gfix_fail(-2, "autotune failed: %s", hipGetErrorString(hipGetLastError()));
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = t;
+  }
+  apply(c, t);
+  c->deriv_ready[0] = c->deriv_ready[1] = 0;  // the fields and partials were used as scratch
+  if (report && cap) {
+    const std::string r = (cached ? std::string("(cached)\n") : std::string()) + t.report;
+    snprintf(report, cap, "%s", r.c_str());
+  }
+  return 0;
+}

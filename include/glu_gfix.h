@@ -195,7 +195,8 @@ int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
 
 /* Per-kernel timing with HIP events on the context's stream.  Kernel
    classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
-   update, 4 line search, 5 gauge update; +6 for the FP32 variants.
+   update, 4 line search, 5 gauge update (FASD), 6 gauge update fused with
+   the next derivative (FACG); +7 for the FP32 variants.
    `mask` selects the classes timed (bit k = class k; -1 all, 0 off).
    profile_read reports the launches that did work (launches issued past
    convergence return at once and are left out), their mean and total. */
@@ -209,6 +210,15 @@ int glu_gfix_profile_read(glu_gfix_ctx *ctx, int kernel, size_t *launches,
    GLU_GFIX_GRAPH environment variable at creation.  Ignored while
    per-kernel profiling is on. */
 int glu_gfix_set_graph(glu_gfix_ctx *ctx, int on);
+
+/* Launch-shape autotuner: times the candidate shapes of this context's
+   kernels on its own scratch fields -- pencils per workgroup of each FFT
+   axis pass, the fused x-y plane pass against two axis passes, the
+   XCD-contiguous workgroup order of the stencil kernels -- and keeps the
+   fastest.  Results are cached per (dims, gftype, precision) in the
+   process.  The choices change speed only.  `report` (may be NULL)
+   receives the timings, one line per candidate.  Call between runs. */
+int glu_gfix_autotune(glu_gfix_ctx *ctx, char *report, size_t cap);
 
 /* Serving loop for streams of requests (GLU fixes one configuration per
    call, Wrappers/GF_wrap.c:179-212; this keeps its memory check,

@@ -232,3 +232,23 @@ def test_xcd_block_order_is_bitwise_neutral(precision, monkeypatch):
             out.append((it, th, g.get_links()))
     assert out[0][:2] == out[1][:2]
     assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_autotune_changes_speed_only():
+    """The tuned launch shapes compute the same transforms (to rounding)
+    and the tuned context still solves; a second context of the same shape
+    reuses the cached choices."""
+    dims = (8, 8, 16, 32)
+    U, plaq = heatbath_config(dims, sweeps=10)
+    with fresh(dims, U) as g:
+        _, F = g.stage_deriv()
+        before = g.stage_fa(F)
+        rep = g.autotune()
+        assert "fft axis" in rep and "stencil order" in rep
+        after = g.stage_fa(F)
+        assert np.abs(after - before).max() < 1e-14 * np.abs(before).max()
+        it, th = g.run(gl.CG, 0.08, 1e-14, 1000)
+        assert th < 1e-14 and it < 1000
+        assert abs(g.measure()["plaquette"] - plaq) < 1e-14
+    with fresh(dims, U) as g:
+        assert g.autotune().startswith("(cached)")
