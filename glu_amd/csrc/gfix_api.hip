@@ -31,7 +31,6 @@ extern "C" const char *glu_gfix_last_error(void) { return g_err.c_str(); }
 namespace {
 
 constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-455
-constexpr double kSwitchTheta = 1e-7;  // mixed precision: FP32 until theta < 1e-7
 constexpr long long kBatch = 8;        // iterations enqueued between host checks
 constexpr int kGluFailures = 7;        // GF_GLU_FAILURES, Headers/GLU_definitions.h:400-402
 
@@ -419,10 +418,10 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
 // one FASD iteration (Gfix/FACG.c:202-235, fixed step GF_TUNE)
 template <typename T> static void enq_sd_iter(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
-  launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s);
+  GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
-  launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 1, s);
+  GFIX_PROF(c, pk<T>(PK_EGAUGE), launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 1, s));
   launch_iter_end(c->ctrl, c->trace, 1, s);
 }
 

@@ -59,7 +59,8 @@ def test_trajectory_first_20(run):
     n = 21
     for col, key in enumerate(("trace_theta", "trace_alpha", "trace_beta")):
         o1, o2, gg = r[1][key][:n], r[NTH][key][:n], tr[:n, col]
-        tol = np.maximum(1e-12 * np.abs(o1), 20 * np.abs(o1 - o2))
+        spread = np.maximum.accumulate(rel(o2, o1)) * np.abs(o1)
+        tol = np.maximum(1e-12 * np.abs(o1), 20 * spread)
         bad = np.abs(gg - o1) > tol
         assert not bad.any(), (key, np.nonzero(bad)[0], gg[bad], o1[bad])
 
