@@ -113,8 +113,8 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     return gfix_fail(-1, "lattice volume %lld is beyond the supported 2^31 - 1 sites", vol);
   }
   geo_set_div(c->g);
-  c->g.xcd = 1;
-  if (const char *e = getenv("GLU_GFIX_XCD")) c->g.xcd = atoi(e) ? 1 : 0;
+  c->g.xcd = 0;
+  if (const char *e = getenv("GLU_GFIX_XCD")) c->ord_eg = c->ord_ls = std::min(std::max(atoi(e), 0), 2);
   c->gftype = gftype;
   c->precision = precision;
   const long long V = c->g.V;
@@ -303,6 +303,13 @@ extern "C" int glu_gfix_restore(glu_gfix_ctx *c) {
 // ---------------------------------------------------------------------------
 template <typename T> static int pk(int id) { return id + (sizeof(T) == 4 ? PK_COUNT : 0); }
 
+// the geometry with the workgroup order chosen for a kernel class
+Geo geo_order(const glu_gfix_ctx *c, int order) {
+  Geo g = c->g;
+  g.xcd = (order == 2 && !geo_slab_order_ok(g)) ? 1 : order;
+  return g;
+}
+
 // forward transform, p^2 preconditioner, backward transform of the 4 packed
 // fields: over the whole lattice (Landau) or per time slice (Coulomb)
 template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, const int *slice_done) {
@@ -373,7 +380,8 @@ template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
   void *&nxt = sizeof(T) == 4 ? c->U32b : c->U64b;
   const Bufs b = bufs_for<T>(c);
   GFIX_PROF(c, pk<T>(PK_EGDERIV),
-            launch_egauge_deriv<T>(cur, nxt, b.sn, b.G, b.F, c->part, c->g, c->ctrl, c->stream));
+            launch_egauge_deriv<T>(cur, nxt, b.sn, b.G, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl,
+                                   c->stream));
   std::swap(cur, nxt);
   c->upar[tix<T>()] ^= 1;
   c->deriv_ready[tix<T>()] = 1;
@@ -386,10 +394,10 @@ template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
 template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
   launch_begin_segment(c->ctrl, 1, s);
-  launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s);
+  launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, s);
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
-  launch_line_search<T>(b.U, b.F, c->part, c->g, c->ctrl, 1, s);
+  launch_line_search<T>(b.U, b.F, c->part, geo_order(c, c->ord_ls), c->ctrl, 1, s);
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 1, 0.15, 0.30, s);
   launch_copy3<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s);
   launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
@@ -401,7 +409,7 @@ template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
 template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
   if (!c->deriv_ready[tix<T>()]) {
-    GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
+    GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, s));
     c->deriv_ready[tix<T>()] = 1;
   }
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
@@ -409,7 +417,7 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
   launch_fin_pr(c->part, c->nblk, c->ctrl, s);
   GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s));
-  GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, c->g, c->ctrl, 0, s));
+  GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, geo_order(c, c->ord_ls), c->ctrl, 0, s));
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
   enq_update_deriv<T>(c);
   launch_iter_end(c->ctrl, c->trace, 0, s);
@@ -418,7 +426,7 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
 // one FASD iteration (Gfix/FACG.c:202-235, fixed step GF_TUNE)
 template <typename T> static void enq_sd_iter(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
-  GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, s));
+  GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, s));
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   GFIX_PROF(c, pk<T>(PK_EGAUGE), launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 1, s));
@@ -511,7 +519,7 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
   // same link buffer it started on)
   if (c->run.method == GLU_GFIX_CG && !c->deriv_ready[tix<T>()]) {
     const Bufs b = bufs_for<T>(c);
-    launch_deriv<T>(b.U, b.F, c->part, c->g, c->ctrl, c->stream);
+    launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, c->stream);
     c->deriv_ready[tix<T>()] = 1;
   }
   const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1) + 4 * c->upar[tix<T>()];

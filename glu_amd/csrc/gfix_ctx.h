@@ -52,6 +52,9 @@ struct glu_gfix_ctx {
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
   double switch_theta = 1e-7;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
+  // workgroup order (Geo::xcd) of the derivative / fused update and of the
+  // line search; set by the autotuner
+  int ord_eg = 1, ord_ls = 1;
   int use_graph = 0;  // replay iteration batches from captured HIP graphs
   hipGraphExec_t graph[8] = {};  // CG/SD x FP64/FP32 x link-buffer parity
   hipEvent_t run_ev = nullptr;
@@ -107,4 +110,5 @@ int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_it
 int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters);
 int landau_advance(glu_gfix_ctx *c, int block);
 int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters);
+gfix::Geo geo_order(const glu_gfix_ctx *c, int order);
 extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed);

@@ -25,10 +25,21 @@ struct Geo {
   // invariant integers using multiplication", 1994, Thm 4.2 with N = 31)
   unsigned dm[4];
   int ds[4];
-  int xcd;  // 1: XCD-contiguous block order in the stencil kernels
+  int xcd;  // stencil block order: 0 plain, 1 XCD-contiguous, 2 XCD t-slabs, t fastest
+  int bpt;  // 256-site blocks per time slice (order 2)
+  int t8;   // time slices per XCD slab (order 2)
 };
 
+// block order 2 needs whole 256-site blocks per time slice and 8 | T
+inline bool geo_slab_order_ok(const Geo &g) { return g.S[3] % 256 == 0 && g.L[3] % 8 == 0; }
+
 inline void geo_set_div(Geo &g) {
+  if (geo_slab_order_ok(g)) {
+    g.bpt = (int)(g.S[3] / 256);
+    g.t8 = g.L[3] / 8;
+  } else {
+    g.bpt = g.t8 = 1;
+  }
   for (int mu = 0; mu < 4; mu++) {
     const unsigned long long d = (unsigned long long)g.L[mu];
     int l = 0;
@@ -49,11 +60,20 @@ __device__ __forceinline__ unsigned geo_div(const Geo &g, int mu, unsigned n) {
 // fetched into that XCD's L2 by a block just before.  Partial sums are
 // stored by logical block, so the reduction order -- and every result --
 // does not depend on the mapping.
+// Order 2 gives XCD k the time slices [k T/8, (k+1) T/8) and walks them
+// with t fastest, so a block's +-t neighbours are the blocks dispatched
+// just before and after it on the same XCD, and its +-z neighbours are
+// (T/8) Lx Ly / 256 blocks away (usable when 256 | Lx Ly Lz and 8 | T).
 __device__ __forceinline__ unsigned logical_block(const Geo &g) {
   const unsigned b = blockIdx.x;
   if (!g.xcd) return b;
-  const unsigned nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, k = b >> 3;
-  return x * q + (x < r ? x : r) + k;
+  const unsigned x = b & 7u, k = b >> 3;
+  if (g.xcd == 2) {
+    const unsigned t8 = (unsigned)g.t8, tl = k % t8, r = k / t8;
+    return (x * t8 + tl) * (unsigned)g.bpt + r;
+  }
+  const unsigned nb = gridDim.x, q = nb >> 3, rm = nb & 7u;
+  return x * q + (x < rm ? x : rm) + k;
 }
 
 __device__ __forceinline__ void coords(const Geo &g, long long i, int x[4]) {

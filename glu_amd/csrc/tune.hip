@@ -3,7 +3,8 @@
 // The tunable shapes are those that change speed and nothing else:
 //   - pencils per workgroup of each LDS FFT axis pass (three variants),
 //   - the fused x-y plane pass against two separate axis passes,
-//   - the XCD-contiguous workgroup order of the stencil kernels.
+//   - the workgroup order of the fused update and of the line search
+//     (plain, XCD-contiguous, XCD time slabs).
 // glu_gfix_autotune times every candidate with HIP events on the context's
 // own scratch fields (the FFT passes and the derivative write only the
 // field planes and the partial sums) and keeps the fastest.  The choices
@@ -25,7 +26,7 @@ namespace {
 struct Tuning {
   int pv[4];
   int use_xy;
-  int xcd;
+  int ord_eg, ord_ls;
   std::string report;
 };
 
@@ -92,24 +93,51 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
              best_axis[0] + best_axis[1], t.use_xy ? "plane" : "axes");
     t.report += line;
   }
-  // workgroup order of the stencil kernels, timed on the derivative
-  float ord[2];
-  for (int x = 0; x < 2; x++) {
-    Geo g = c->g;
-    g.xcd = x;
-    ord[x] = time_ms(c, [&] { launch_deriv<T>(U, F, c->part, g, nullptr, c->stream); }, kReps);
+  // workgroup order of the two stencil kernels that dominate an iteration,
+  // each timed on its own with a private control block (not converged,
+  // fixed step): the fused update writes only scratch (the second link
+  // buffer, the fields, the partials, and G, which a run resets)
+  Ctrl h{};
+  h.alpha = 0.1;
+  h.theta = 1.0;
+  h.cg_tol = 5e-11;
+  Ctrl *dc = nullptr;
+  if (hipMalloc(&dc, sizeof(Ctrl)) == hipSuccess &&
+      hipMemcpyAsync(dc, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream) == hipSuccess) {
+    void *Unext = sizeof(T) == 4 ? c->U32b : c->U64b;
+    void *sn = sizeof(T) == 4 ? c->sn32 : c->sn;
+    void *G = sizeof(T) == 4 ? c->G64 : nullptr;
+    const int norder = geo_slab_order_ok(c->g) ? 3 : 2;
+    const char *names[3] = {"plain", "XCD-contiguous", "XCD t-slabs"};
+    float eg[3] = {1e30f, 1e30f, 1e30f}, ls[3] = {1e30f, 1e30f, 1e30f};
+    for (int o = 0; o < norder; o++) {
+      const Geo g = geo_order(c, o);
+      eg[o] = time_ms(c, [&] { launch_egauge_deriv<T>(U, Unext, sn, G, F, c->part, g, dc, c->stream); }, kReps);
+      ls[o] = time_ms(c, [&] { launch_line_search<T>(U, sn, c->part, g, dc, 1, c->stream); }, kReps);
+      snprintf(line, sizeof(line), "order %s: gauge update + derivative %.4f ms, line search %.4f ms\n", names[o],
+               eg[o], ls[o]);
+      t.report += line;
+    }
+    t.ord_eg = t.ord_ls = 0;
+    for (int o = 1; o < norder; o++) {
+      if (eg[o] < eg[t.ord_eg]) t.ord_eg = o;
+      if (ls[o] < ls[t.ord_ls]) t.ord_ls = o;
+    }
+    snprintf(line, sizeof(line), "stencil order: update %s, line search %s\n", names[t.ord_eg], names[t.ord_ls]);
+    t.report += line;
+  } else {
+    t.ord_eg = t.ord_ls = 1;
   }
-  t.xcd = ord[1] <= ord[0];
-  snprintf(line, sizeof(line), "stencil order: plain %.4f ms, XCD-contiguous %.4f ms -> %s\n", ord[0], ord[1],
-           t.xcd ? "XCD-contiguous" : "plain");
-  t.report += line;
+  (void)hipStreamSynchronize(c->stream);
+  if (dc) (void)hipFree(dc);
   return t;
 }
 
 void apply(glu_gfix_ctx *c, const Tuning &t) {
   for (int a = 0; a < 4; a++) c->ax[a].pv = t.pv[a];
   c->use_xy = t.use_xy;
-  c->g.xcd = t.xcd;
+  c->ord_eg = t.ord_eg;
+  c->ord_ls = t.ord_ls;
   for (hipGraphExec_t &gx : c->graph)  // captured launches used the old shapes
     if (gx) {
       (void)hipGraphExecDestroy(gx);

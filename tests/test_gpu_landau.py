@@ -223,19 +223,20 @@ def test_unit_and_pure_gauge():
 
 @pytest.mark.parametrize("precision", [gl.FP64, gl.MIXED])
 def test_xcd_block_order_is_bitwise_neutral(precision, monkeypatch):
-    """The XCD-contiguous block order changes which workgroup computes a
-    block, not what it computes or where its partial sum goes: whole runs
-    agree bitwise with the plain order."""
+    """The workgroup orders (plain, XCD-contiguous, XCD time slabs) change
+    which workgroup computes a block, not what it computes or where its
+    partial sum goes: whole runs agree bitwise."""
     dims = (8, 8, 8, 16)
     U, _ = heatbath_config(dims, sweeps=10)
     out = []
-    for xcd in ("0", "1"):
+    for xcd in ("0", "1", "2"):
         monkeypatch.setenv("GLU_GFIX_XCD", xcd)
         with fresh(dims, U, precision) as g:
             it, th = g.run(gl.CG, 0.08, 1e-14, 1000)
             out.append((it, th, g.get_links()))
-    assert out[0][:2] == out[1][:2]
-    assert np.array_equal(out[0][2], out[1][2])
+    for o in out[1:]:
+        assert out[0][:2] == o[:2]
+        assert np.array_equal(out[0][2], o[2])
 
 
 def test_autotune_changes_speed_only():
