@@ -45,7 +45,7 @@ ALG_ITER = (3136, 1568)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dims", default="32,32,32,64")
     ap.add_argument("--sweeps", type=int, default=200)

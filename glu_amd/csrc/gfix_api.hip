@@ -150,6 +150,8 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
       CK(hipMalloc(&c->sn32, (size_t)V * 4 * sizeof(float2)));
       CK(hipMalloc(&c->old32, (size_t)V * 4 * sizeof(float2)));
       if (const char *e = getenv("GLU_GFIX_SWITCH_THETA")) c->switch_theta = atof(e);
+      if (const char *e = getenv("GLU_GFIX_SAFEGUARD")) c->sg_mode = atoi(e);
+      if (const char *e = getenv("GLU_GFIX_BETA_CAP")) c->beta_cap = atof(e);
     }
     // FFT axes, twiddles and the p^2 cos tables
     const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
@@ -492,6 +494,9 @@ static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, siz
     h.trace_n = c->trace_n;
   }
   h.switch_theta = switch_theta;
+  h.sg_mode = c->sg_mode;
+  h.beta_cap = c->beta_cap;
+  h.sg_uphill = 0;
   h.max_iters = (long long)max_iters;
   h.done = 1;
   r.max_phase = max_iters;

@@ -51,6 +51,11 @@ struct glu_gfix_ctx {
   // FP32 packed fields of the mixed schedule (in, sn, in_old)
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
   double switch_theta = 1e-7;
+  // FP32-phase safeguards (Ctrl::sg_mode): beta capped at 3, chosen over
+  // 20 gauge copies of a 32^3x64 beta = 6.0 config; the FP64 phase and
+  // FP64 runs keep GLU's beta
+  int sg_mode = 2;
+  double beta_cap = 3.0;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner

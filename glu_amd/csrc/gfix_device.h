@@ -122,6 +122,13 @@ struct Ctrl {
   int switched;      // mixed precision reached switch_theta
   long long trace_cap;
   long long trace_n;
+  // safeguards of the FP32 phase of a mixed run (active while
+  // switch_theta > 0): bit 0 restarts CG (beta = 0) after an uphill
+  // direction, bit 1 caps beta at beta_cap, bit 2 takes no step along an
+  // uphill direction
+  int sg_mode;
+  int sg_uphill;  // the last line search found the direction uphill
+  double beta_cap;
 };
 
 // ---------------------------------------------------------------- reduce --

@@ -321,6 +321,11 @@ __global__ void k_fin_pr(const double *__restrict__ part, long long nb, Ctrl *ct
     double beta = sc / ctrl->inold;
     if (beta < 0.0) beta = 0.0;
     if (ctrl->theta < ctrl->cg_tol) beta = insum / ctrl->inold;
+    if (ctrl->switch_theta > 0.0) {  // safeguards of the mixed FP32 phase
+      if ((ctrl->sg_mode & 1) && ctrl->sg_uphill) beta = 0.0;
+      if ((ctrl->sg_mode & 2) && beta > ctrl->beta_cap) beta = ctrl->beta_cap;
+    }
+    ctrl->sg_uphill = 0;
     ctrl->inold = insum;
     ctrl->insum = insum;
     ctrl->sum_conj = sc;
@@ -372,6 +377,10 @@ __global__ void k_fin_ls(const double *__restrict__ part, long long nb, Ctrl *ct
       const double al[3] = {0.0, pl1, pl2};
       ctrl->alpha = approx_minimum(al, val);
       ctrl->did_ls = 1;
+      // mixed FP32 phase: the functional rises along the direction
+      const int sg = ctrl->switch_theta > 0.0 && !force ? ctrl->sg_mode : 0;
+      ctrl->sg_uphill = sg && val[1] > val[0];
+      if ((sg & 4) && ctrl->sg_uphill) ctrl->alpha = 0.0;
     } else {
       ctrl->alpha = ctrl->gf_alpha;
       ctrl->did_ls = 0;

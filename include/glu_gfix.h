@@ -158,7 +158,11 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
 
 /* Mixed precision: run FP32 until theta < switch_theta, then FP64 (default
-   1e-7, or the GLU_GFIX_SWITCH_THETA environment variable at creation). */
+   1e-7, or the GLU_GFIX_SWITCH_THETA environment variable at creation).
+   The FP32 phase caps the Polak-Ribiere beta at 3 (GLU_GFIX_BETA_CAP;
+   GLU_GFIX_SAFEGUARD=0 turns it off): near its rounding floor the FP32
+   gradient makes PR+ directions run uphill for hundreds of iterations,
+   and the cap keeps beta bounded there.  The FP64 phase follows GLU. */
 int glu_gfix_set_switch_theta(glu_gfix_ctx *ctx, double switch_theta);
 
 /* Iterations of the last mixed-precision run spent in FP32 / FP64. */
