@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="CPU baseline sample: k and 2k oracle iterations")
     ap.add_argument("--workload", choices=("landau", "coulomb", "serve"), default="landau")
+    ap.add_argument("--graph", action="store_true", help="replay iteration batches from HIP graphs")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--rate", type=float, default=2.0,
                     help="serve: Poisson arrivals per second (about 70%% of the one-at-a-time capacity)")
@@ -145,6 +146,7 @@ def main():
     print(f"[bench] input ready in {time.perf_counter() - t:.1f}s, plaquette {plaq:.12f}",
           file=sys.stderr, flush=True)
     tune = gm.autotune()  # launch shapes, before any timing
+    gm.set_graph(a.graph)
     print("[bench] autotune:\n" + tune.rstrip(), file=sys.stderr, flush=True)
 
     total, lat, its, restarts, (n32, n64), func, prof, dom, dom_live = timed_runs(
@@ -161,6 +163,7 @@ def main():
         g64.set_links(U)
         g64.snapshot()
         g64.autotune()
+        g64.set_graph(a.graph)
         t64, lat64, its64, rs64, _, func64, prof64, _, _ = timed_runs(g64, a.steps, a.warmup, a.seed + 1)
         rel = [abs(x - y) / abs(y) for x, y in zip(func, func64)]
         fp64 = {"configs_per_s": a.steps / t64, "ms_per_step": 1e3 * t64 / a.steps,
@@ -210,7 +213,7 @@ def main():
                  f"(random gauge transform seed {a.seed + 1}+k), with Landau() restarts"),
         "config": {"workload": "landau_facg_mixed_" + "x".join(map(str, dims)),
                    "lattice": list(dims), "gf_tune": 0.08, "accuracy": 1e-14, "max_iters": 1000,
-                   "switch_theta": a.switch_theta},
+                   "switch_theta": a.switch_theta, "graphs": a.graph},
         "p50_ms": round(1e3 * pct(lat, 0.5), 2), "p99_ms": round(1e3 * pct(lat, 0.99), 2),
         "iters": its, "restarts": restarts, "iters_fp32": n32, "iters_fp64": n64,
         "stream_GBps": round(gbs, 1),
