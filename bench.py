@@ -45,7 +45,7 @@ ALG_ITER = (3136, 1568)
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=12)
+    ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--dims", default="32,32,32,64")
     ap.add_argument("--sweeps", type=int, default=200)
@@ -168,8 +168,10 @@ def main():
         g64.set_graph(a.graph)
         t64, lat64, its64, rs64, _, func64, prof64, _, _ = timed_runs(g64, a.steps, a.warmup, a.seed + 1)
         rel = [abs(x - y) / abs(y) for x, y in zip(func, func64)]
+        n64only = sum(i for i in its64 if i < 123456789)
         fp64 = {"configs_per_s": a.steps / t64, "ms_per_step": 1e3 * t64 / a.steps,
                 "iters": its64, "restarts": rs64, "p50_ms": 1e3 * pct(lat64, 0.5),
+                "ms_per_iter": round(1e3 * t64 / max(n64only, 1), 4),
                 # error of the mixed result against the FP64 one, per gauge copy:
                 # ~1e-13 on the same Gribov copy, 1e-6..1e-4 on a neighbouring one
                 "functional_rel_diff_mixed": [float(f"{r:.2e}") for r in rel]}
@@ -218,6 +220,8 @@ def main():
                    "switch_theta": a.switch_theta, "graphs": a.graph},
         "p50_ms": round(1e3 * pct(lat, 0.5), 2), "p99_ms": round(1e3 * pct(lat, 0.99), 2),
         "iters": its, "restarts": restarts, "iters_fp32": n32, "iters_fp64": n64,
+        # kernel speed apart from the (chaotic) iteration counts
+        "ms_per_iter": round(1e3 * total / max(n32 + n64, 1), 4),
         "stream_GBps": round(gbs, 1),
         "fp64": fp64,
         "mixed_speedup_vs_fp64": round(mixed_value / fp64["configs_per_s"], 3) if fp64 else None,

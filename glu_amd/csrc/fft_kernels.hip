@@ -35,6 +35,24 @@ template <typename T, int R> __device__ __forceinline__ void dft_small(cx<T> (&v
     v[2] = csub(a, c);
     v[1] = cadd(b, id);
     v[3] = csub(b, id);
+  } else if constexpr (R == 8) {
+    // two 4-point DFTs of the even and odd inputs, combined with the
+    // eighth roots of unity e^{sign i pi k / 4}
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    dft_small<T, 4>(e, sign);
+    dft_small<T, 4>(o, sign);
+    const T h = T(0.70710678118654752440);
+    const T sg = sign < 0 ? T(-1) : T(1);
+    cx<T> w[4];
+    w[0] = o[0];
+    w[1] = mk<T>(h * (o[1].re - sg * o[1].im), h * (o[1].im + sg * o[1].re));  // * h (1 + i sg)
+    w[2] = mk<T>(-sg * o[2].im, sg * o[2].re);                                 // * i sg
+    w[3] = mk<T>(-h * (o[3].re + sg * o[3].im), h * (sg * o[3].re - o[3].im)); // * h (-1 + i sg)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      v[k] = cadd(e[k], w[k]);
+      v[k + 4] = csub(e[k], w[k]);
+    }
   } else if constexpr (R == 5 || R == 7) {
     // plain O(R^2) DFT for the odd primes GLU's extents may contain
     constexpr double c5[5] = {1.0, 0.30901699437494745, -0.8090169943749475, -0.8090169943749475,
@@ -212,8 +230,14 @@ struct RadPlan {
   int n;
   int r[8];
 };
+// radix 8 first: two stages where radix 4/2 needs three (N = 32, 64), so
+// one LDS round trip and two barriers fewer per pass
 constexpr RadPlan make_plan(int N) {
   RadPlan p{0, {0, 0, 0, 0, 0, 0, 0, 0}};
+  while (N % 8 == 0) {
+    p.r[p.n++] = 8;
+    N /= 8;
+  }
   while (N % 4 == 0) {
     p.r[p.n++] = 4;
     N /= 4;
