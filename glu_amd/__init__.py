@@ -32,7 +32,7 @@ EXPORTS = [
     "glu_gfix_stage_line_search", "glu_gfix_stage_egauge", "glu_gfix_footprint",
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
-    "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune",
+    "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy",
 ]
@@ -81,6 +81,8 @@ def lib():
             "glu_gfix_get_links_device": (C.c_int, [P, C.c_void_p]),
             "glu_gfix_set_graph": (C.c_int, [P, C.c_int]),
             "glu_gfix_autotune": (C.c_int, [P, C.c_char_p, SZ]),
+            "glu_gfix_luxury": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, C.c_int, C.c_uint64, DP,
+                                          C.POINTER(SZ), C.POINTER(C.c_int)]),
             "glu_gfix_server_create": (C.c_int, [C.POINTER(P), SZ, C.c_int]),
             "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int, C.c_uint64]),
             "glu_gfix_landau": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, C.c_uint64, DP,
@@ -181,6 +183,13 @@ class Gfix:
         _check(lib().glu_gfix_landau(self._h, method, gf_tune, accuracy, max_iters, restart_seed,
                                      C.byref(th), C.byref(it), C.byref(nf)), "landau")
         return it.value, th.value, nf.value
+
+    def luxury(self, method=CG, gf_tune=0.08, accuracy=1e-14, max_iters=1000, copies=20, seed=1):
+        """LUXURY_GAUGE: best of `copies` Gribov copies.  Returns (iters, theta, chosen copy)."""
+        th, it, best = C.c_double(), SZ(), C.c_int()
+        _check(lib().glu_gfix_luxury(self._h, method, gf_tune, accuracy, max_iters, copies, seed,
+                                     C.byref(th), C.byref(it), C.byref(best)), "luxury")
+        return it.value, th.value, best.value
 
     def trace(self, cap=16384) -> np.ndarray:
         rows = np.zeros((cap, 3))

@@ -204,7 +204,7 @@ bad:
 extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
+  void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
                   c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -720,6 +720,57 @@ extern "C" int glu_gfix_landau(glu_gfix_ctx *c, int method, double gf_tune, doub
   if (failures) *failures = fail;
   *iters = it;
   if (fail == kGluFailures) return gfix_fail(-5, "no convergence after %d random restarts (GF_TUNE %g)", fail - 1, gf_tune);
+  return 0;
+}
+
+// luxury_copy_fast of Gfix/Landau.c:88-213 (LUXURY_GAUGE, with BEST_COPY
+// as Headers/GLU_definitions.h:441-443 sets it): each copy is the input
+// under a random gauge transform, fixed to 1e-6 (at most 1000 iterations);
+// the copy with the smallest functional among those that neither failed
+// nor used all 1000 iterations is fixed to `accuracy`.  The count is the
+// last copy's iterations plus the final run's, as in GLU.  Copy k uses the
+// device transform seeded seed + k (GLU draws from its MWC streams).  If
+// no copy qualifies GLU continues from an unset buffer; here the final run
+// starts from the input.
+extern "C" int glu_gfix_luxury(glu_gfix_ctx *c, int method, double gf_tune, double accuracy, size_t max_iters,
+                               int copies, uint64_t seed, double *theta, size_t *iters, int *best_copy) {
+  if (!c || !theta || !iters) return gfix_fail(-1, "glu_gfix_luxury: null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "glu_gfix_luxury: not a Landau context");
+  if (method != GLU_GFIX_CG && method != GLU_GFIX_SD) return gfix_fail(-1, "unknown method %d", method);
+  if (max_iters == 0 || copies < 1) return gfix_fail(-1, "max_iters and copies must be positive");
+  const size_t bytes = (size_t)c->g.V * 24 * sizeof(double2);
+  if (!c->Uorig && hipMalloc(&c->Uorig, bytes) != hipSuccess) return gfix_fail(-2, "input copy allocation failed");
+  if (!c->Ubest && hipMalloc(&c->Ubest, bytes) != hipSuccess) return gfix_fail(-2, "best copy allocation failed");
+  if (hipMemcpyAsync(c->Uorig, c->U64, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(c->Ubest, c->U64, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    return gfix_fail(-2, "input copy failed");
+  double best = 1.0;  // GLU's maxlink under BEST_COPY
+  int chosen = -1;
+  size_t it = 0;
+  for (int k = 0; k < copies; k++) {
+    if (hipMemcpyAsync(c->U64, c->Uorig, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      return gfix_fail(-2, "input reload failed");
+    if (int rc = glu_gfix_random_transform(c, seed + (uint64_t)k)) return rc;
+    double th = 0.0, f = 0.0;
+    if (int rc = landau_fix(c, method, gf_tune, 1e-6, 1000, &th, &it)) return rc;
+    if (int rc = glu_gfix_measure(c, nullptr, nullptr, nullptr, &f, nullptr)) return rc;
+    if (c->verbose) fprintf(stdout, "  [COPY] %d [FUNCTIONAL] %1.15f [ITER] %zu ", k, f, it);
+    if (f < best && it != 1000 && it != 123456789) {
+      best = f;
+      chosen = k;
+      if (hipMemcpyAsync(c->Ubest, c->U64, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        return gfix_fail(-2, "best copy failed");
+      if (c->verbose) fprintf(stdout, " -> Copy accepted\n");
+    } else if (c->verbose) {
+      fprintf(stdout, " -> Copy rejected %e\n", f - best);
+    }
+  }
+  if (hipMemcpyAsync(c->U64, c->Ubest, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    return gfix_fail(-2, "best copy reload failed");
+  size_t fin = 0;
+  if (int rc = landau_fix(c, method, gf_tune, accuracy, max_iters, theta, &fin)) return rc;
+  *iters = it + fin;
+  if (best_copy) *best_copy = chosen;
   return 0;
 }
 

@@ -40,6 +40,7 @@ struct glu_gfix_ctx {
   // FP32 accumulated site transform for the mixed schedule, snapshot
   void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
   void *Uorig = nullptr;  // input links kept for the Landau() restarts
+  void *Ubest = nullptr;  // best Gribov copy so far (glu_gfix_luxury)
   // second link buffers: the CG gauge update writes out of place and the
   // host swaps U64 <-> U64b (U32 <-> U32b) as it enqueues it; upar counts
   // the swaps mod 2 (FP64, FP32) so captured graphs know their start
@@ -117,3 +118,4 @@ int landau_advance(glu_gfix_ctx *c, int block);
 int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters);
 gfix::Geo geo_order(const glu_gfix_ctx *c, int order);
 extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed);
+extern "C" int glu_gfix_measure(glu_gfix_ctx *c, double *plaq, double *tr, double *mx, double *func, double *tlink);

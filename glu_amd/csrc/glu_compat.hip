@@ -378,13 +378,27 @@ void output_fixing_info(glu_gfix_ctx *c, struct site *lat, double theta, size_t 
   printf("[GF] Functional :: %1.15f\n", func);
 }
 
-size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int method) {
+// LUXURY_GAUGE copies requested at run time (GLU_GFIX_LUXURY=N; a value
+// that is not a positive number means GLU's default of 20, configure.ac:493)
+int luxury_copies() {
+  const char *e = getenv("GLU_GFIX_LUXURY");
+  if (!e) return 0;
+  const int n = atoi(e);
+  return n > 0 ? n : 20;
+}
+
+// FA_callback of Gfix/Landau.c:225-237: FACG, FASD, or luxury_copy_fast
+size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int method, bool callback) {
   glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
   if (!c) return 123456789;
   std::vector<double> h;
   pack(lat, h);
   size_t iters = 0;
-  if (glu_gfix_set_links(c, h.data()) || glu_gfix_run(c, method, Latt.gf_alpha, acc, max_iters, th, &iters)) {
+  const int lux = callback ? luxury_copies() : 0;
+  const uint64_t seed = Latt.Seed[0] ? (uint64_t)Latt.Seed[0] : 1;
+  if (glu_gfix_set_links(c, h.data()) ||
+      (lux ? glu_gfix_luxury(c, method, Latt.gf_alpha, acc, max_iters, lux, seed, th, &iters, nullptr)
+           : glu_gfix_run(c, method, Latt.gf_alpha, acc, max_iters, th, &iters))) {
     fprintf(stderr, "[GF] device gauge fixing failed: %s\n", glu_gfix_last_error());
     return 123456789;
   }
@@ -397,19 +411,19 @@ size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int m
 
 extern "C" size_t FACG(struct site *lat, struct fftw_stuff *FFTW, double *th, const double acc,
                        const size_t max_iters) {
-  return fa_call(lat, th, acc, max_iters, GLU_GFIX_CG);
+  return fa_call(lat, th, acc, max_iters, GLU_GFIX_CG, false);
 }
 
 extern "C" size_t FASD(struct site *lat, struct fftw_stuff *FFTW, double *th, const double acc,
                        const size_t max_iters) {
-  return fa_call(lat, th, acc, max_iters, GLU_GFIX_SD);
+  return fa_call(lat, th, acc, max_iters, GLU_GFIX_SD, false);
 }
 
 // Gfix/Landau.c:261-374 (the FACG callback; FASD with GLU_GFIX_SD)
 extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t max_iter, const char *infile) {
   const int method = getenv("GLU_GFIX_SD") ? GLU_GFIX_SD : GLU_GFIX_CG;
   double theta = 0.0;
-  size_t iters = fa_call(lat, &theta, accuracy, max_iter, method);
+  size_t iters = fa_call(lat, &theta, accuracy, max_iter, method, true);
   size_t failure = 0;
   if (iters == 123456789) {
     fprintf(stderr, "\n[GF] Non-convergence ... Randomly-restarting \n\n[GF] Failure :: %zu || Iters :: %zu || ACC :: %e \n",
@@ -422,7 +436,7 @@ extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t m
       }
       random_transform(h);
       unpack(lat, h);
-      const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method);
+      const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method, true);
       if (loc != 123456789) {
         iters += loc;
         break;

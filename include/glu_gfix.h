@@ -140,6 +140,19 @@ int glu_gfix_landau(glu_gfix_ctx *ctx, int method, double gf_tune,
                     double accuracy, size_t max_iters, uint64_t restart_seed,
                     double *theta, size_t *iters, int *failures);
 
+/* luxury_copy_fast of Gfix/Landau.c:88-213 (the LUXURY_GAUGE build, with
+   BEST_COPY): `copies` random gauge copies of the device links, each
+   fixed to 1e-6 in at most 1000 iterations; the copy with the smallest
+   functional among those that converged is then fixed to `accuracy`.
+   *iters = last copy's iterations + the final run's, as GLU counts.  Copy
+   k applies glu_gfix_random_transform(ctx, seed + k).  *best_copy (may be
+   NULL): the chosen copy, -1 if none qualified (the final run then starts
+   from the input). */
+int glu_gfix_luxury(glu_gfix_ctx *ctx, int method, double gf_tune,
+                    double accuracy, size_t max_iters, int copies,
+                    uint64_t seed, double *theta, size_t *iters,
+                    int *best_copy);
+
 /* Trajectory of the last run: rows (theta_k, alpha_k, beta_k); row 0 is the
    steepest-descent start.  Landau only.  Recording costs nothing extra. */
 int glu_gfix_trace(glu_gfix_ctx *ctx, double *rows, size_t cap, size_t *n);

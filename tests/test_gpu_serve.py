@@ -174,3 +174,36 @@ def test_server_reports_failure_and_keeps_serving():
         it, th, _ = srv.wait(t_good)
     assert (it, th) == (it0, th0)
     assert np.array_equal(good.cpu().numpy(), W)
+
+
+def test_luxury_copies_pick_the_smallest_functional():
+    """glu_gfix_luxury against luxury_copy_fast (Gfix/Landau.c:88-213,
+    BEST_COPY) replayed by hand: copy k is the input under the transform
+    seeded seed + k, fixed to 1e-6 in at most 1000 iterations; the smallest
+    functional among converged copies is then fixed to the accuracy."""
+    dims = (8, 8, 8, 8)
+    U, _ = heatbath_config(dims, sweeps=20)
+    seed, copies = 11, 4
+    with gl.Gfix(dims, gl.LANDAU, gl.FP64) as g:
+        g.set_links(U)
+        g.snapshot()
+        funcs, its = [], []
+        for k in range(copies):
+            g.restore()
+            g.random_transform(seed + k)
+            it, _ = g.run(gl.CG, 0.08, 1e-6, 1000)
+            funcs.append(g.measure()["functional"])
+            its.append(it)
+        ok = [k for k in range(copies) if its[k] not in (1000, 123456789)]
+        best = min(ok, key=lambda k: funcs[k])
+        g.restore()
+        g.random_transform(seed + best)
+        g.run(gl.CG, 0.08, 1e-6, 1000)
+        fin, th_ref = g.run(gl.CG, 0.08, 1e-14, 1000)
+        W_ref = g.get_links()
+        g.restore()
+        it, th, chosen = g.luxury(gl.CG, 0.08, 1e-14, 1000, copies=copies, seed=seed)
+        assert chosen == best
+        assert it == its[-1] + fin and th == th_ref
+        assert np.array_equal(g.get_links(), W_ref)
+        assert len(set(np.round(funcs, 12))) > 1  # the copies differ
