@@ -115,7 +115,7 @@ def oracle_run(U, dims, nth, **kw):
     return L, W, r
 
 
-@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (8, 8, 8, 16)])
+@pytest.mark.parametrize("dims", [(8, 8, 8, 8), (8, 8, 8, 16), (6, 10, 6, 12)])
 def test_facg_trajectory_and_final(dims):
     U, plaq = heatbath_config(dims)
     L, W1, r1 = oracle_run(U, dims, 1)
