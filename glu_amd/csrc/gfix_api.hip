@@ -372,7 +372,7 @@ template <typename T> static Bufs bufs_for(glu_gfix_ctx *c) {
   return {c->U64, nullptr, c->F, c->sn, c->old};
 }
 
-// steepest-descent start of a FACG segment (Gfix/FACG.c:249-265)
+// index of a precision in the per-precision state (0 FP64, 1 FP32)
 template <typename T> static constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
 
 // the CG gauge update, out of place and fused with the next derivative;

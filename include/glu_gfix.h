@@ -269,7 +269,8 @@ int glu_gfix_server_wait(glu_gfix_server *srv, int ticket, double *theta,
 /* Completed requests, nearest-rank p50/p99 latency, peak bytes held. */
 int glu_gfix_server_stats(glu_gfix_server *srv, size_t *completed,
                           double *p50_ms, double *p99_ms, size_t *peak_bytes);
-/* Finishes every submitted request, then frees the pooled contexts. */
+/* Finishes every submitted request, then frees the pooled contexts.  Call
+   it once no thread is inside glu_gfix_server_wait. */
 int glu_gfix_server_destroy(glu_gfix_server *srv);
 
 /* Bytes of device memory a context of these dims needs. */
