@@ -47,6 +47,26 @@ constexpr double PC1 = 0.17, PC2 = 0.34;  // Gfix/line_search.c:30-31
 
 __device__ __forceinline__ long long site_of(const Geo &g, long long t, long long xl) { return t * g.Lcu + xl; }
 
+// (slice, block in slice) of this workgroup of a (blocks per slice, T)
+// grid.  The dispatcher deals workgroups to the 8 XCDs round-robin.  With
+// g.xcd set (and 8 | blocks per slice) XCD x is given the contiguous eighth
+// x of every slice instead, so a block's +-z neighbours (Lx Ly / 256 blocks
+// away) were fetched into the same L2 just before, while every XCD keeps a
+// share of every slice (slices stop at different iterations; whole slices
+// per XCD would leave some XCDs idle).  Partials are indexed by (slice,
+// block), so results do not depend on the order.
+__device__ __forceinline__ void slice_block(const Geo &g, int &t, long long &bx) {
+  const unsigned l = blockIdx.x + blockIdx.y * gridDim.x, nbs = gridDim.x;
+  if (g.xcd && (nbs & 7u) == 0) {
+    const unsigned e = nbs >> 3, x = l & 7u, k = l >> 3;
+    t = (int)(k / e);
+    bx = (long long)(x * e + k % e);
+    return;
+  }
+  t = (int)(l / nbs);
+  bx = (long long)(l % nbs);
+}
+
 __device__ __forceinline__ void herm_short(const su3<double> &U, double s[10]) {
   // Matrix_Ops/exactQ.c:400-428: {Q00, Q01, Q02, Q11, Q12}
   const double i0 = U.e[0].im, i4 = U.e[4].im, i8 = U.e[8].im;
@@ -66,9 +86,11 @@ __device__ __forceinline__ void herm_short(const su3<double> &U, double s[10]) {
 // derivative of the gauge-rotated spatial links: Gfix/CFACG.c:92-163
 __global__ __launch_bounds__(kBlock) void k_cderiv(const double2 *__restrict__ U, const double2 *__restrict__ G,
                                                   double2 *__restrict__ F, Geo g, SliceCtrl sc) {
-  const int t = blockIdx.y;
+  int t;
+  long long bx;
+  slice_block(g, t, bx);
   if (sc.done[t]) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
   const long long i = site_of(g, t, xl), V = g.V;
   int x[4];
@@ -95,9 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_cderiv(const double2 *__restrict__ U
 // trace-metric sums per slice (Gfix/CFACG.c:166-260)
 __global__ __launch_bounds__(kBlock) void k_cpr(const double2 *__restrict__ F, const double2 *__restrict__ O,
                                                double *__restrict__ part, Geo g, SliceCtrl sc, long long nbs) {
-  const int t = blockIdx.y;
+  int t;
+  long long bx;
+  slice_block(g, t, bx);
   if (sc.done[t]) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long xl = bx * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
   if (xl < g.Lcu) {
     const long long i = site_of(g, t, xl), V = g.V;
@@ -113,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_cpr(const double2 *__restrict__ F, c
     v[0] = s1;
     v[1] = s2;
   }
-  block_sum_store<2>(v, part + 2 * (t * nbs + blockIdx.x));
+  block_sum_store<2>(v, part + 2 * (t * nbs + bx));
 }
 
 __device__ double block_reduce_1024(double v, double *sh) {
@@ -159,9 +183,11 @@ __global__ void k_cfin_pr(const double *__restrict__ part, long long nbs, SliceC
 // sn = in + beta sn ; old = in  (mode 0: sn = old = in)
 __global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ F, double2 *__restrict__ sn,
                                                    double2 *__restrict__ O, Geo g, SliceCtrl sc, int mode) {
-  const int t = blockIdx.y;
+  int t;
+  long long bx;
+  slice_block(g, t, bx);
   if (sc.done[t]) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
   const long long i = site_of(g, t, xl), V = g.V;
   const double beta = mode ? sc.beta[t] : 0.0;
@@ -188,10 +214,12 @@ __device__ __forceinline__ su3<double> cgauge_at(const double2 *__restrict__ F, 
 __global__ __launch_bounds__(kBlock) void k_cline(const double2 *__restrict__ U, const double2 *__restrict__ G,
                                                  const double2 *__restrict__ F, double *__restrict__ part, Geo g,
                                                  SliceCtrl sc, long long nbs, int force) {
-  const int t = blockIdx.y;
+  int t;
+  long long bx;
+  slice_block(g, t, bx);
   if (sc.done[t]) return;
   if (!force && !(sc.theta[t] > sc.cg_tol)) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long xl = bx * blockDim.x + threadIdx.x;
   double v[3] = {0.0, 0.0, 0.0};
   if (xl < g.Lcu) {
     const long long i = site_of(g, t, xl), V = g.V;
@@ -208,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_cline(const double2 *__restrict__ U,
       v[2] += retrace_abc_dag<double>(A2, C, cgauge_at(F, V, fw, PC2));
     }
   }
-  block_sum_store<3>(v, part + 3 * (t * nbs + blockIdx.x));
+  block_sum_store<3>(v, part + 3 * (t * nbs + bx));
 }
 
 __device__ __forceinline__ double approx_min(const double al[3], const double f[3]) {
@@ -252,9 +280,11 @@ __global__ void k_cfin_ls(const double *__restrict__ part, long long nbs, SliceC
 // G(x) <- P(1 + alpha_t X(F(x))) G(x): Gfix/line_search.c:101-115
 __global__ __launch_bounds__(kBlock) void k_cexp(double2 *__restrict__ G, const double2 *__restrict__ F, Geo g,
                                                 SliceCtrl sc) {
-  const int t = blockIdx.y;
+  int t;
+  long long bx;
+  slice_block(g, t, bx);
   if (sc.done[t]) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
   const long long i = site_of(g, t, xl), V = g.V;
   const su3<double> a = cgauge_at(F, V, i, sc.alpha[t]);
@@ -416,7 +446,8 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
   SliceCtrl sc = cs->sc;
   const int T = c->g.L[3];
   hipStream_t s = c->stream;
-  const Geo g = c->g;
+  Geo g = c->g;
+  g.xcd = c->ord_cl ? 1 : 0;
   const dim3 grid((unsigned)cs->nbs, T), blk(kBlock);
   std::vector<int> done(T);
   std::vector<long long> zero(T, 0);

@@ -114,7 +114,10 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
   }
   geo_set_div(c->g);
   c->g.xcd = 0;
-  if (const char *e = getenv("GLU_GFIX_XCD")) c->ord_eg = c->ord_ls = std::min(std::max(atoi(e), 0), 2);
+  if (const char *e = getenv("GLU_GFIX_XCD")) {
+    c->ord_eg = c->ord_ls = std::min(std::max(atoi(e), 0), 2);
+    c->ord_cl = c->ord_eg ? 1 : 0;
+  }
   c->gftype = gftype;
   c->precision = precision;
   const long long V = c->g.V;

@@ -61,6 +61,7 @@ struct glu_gfix_ctx {
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner
   int ord_eg = 1, ord_ls = 1;
+  int ord_cl = 1;  // Coulomb slice stencils: XCD-contiguous (1) or plain (0)
   int use_graph = 0;  // replay iteration batches from captured HIP graphs
   hipGraphExec_t graph[8] = {};  // CG/SD x FP64/FP32 x link-buffer parity
   hipEvent_t run_ev = nullptr;

@@ -54,8 +54,13 @@ def test_conf_fixture_matches_oracle():
     sp_g, t_g = links_sp_t(Wg)
     assert rel(sp_g, sp_o) < 1e-10 and rel(t_g, t_o) < 1e-8
     assert abs(m["plaquette"] - L.plaquette(U)) < 1e-13
+    # per-link traces: a solve ending at theta < 1e-14 fixes them to about
+    # 1e-10, the oracle's own spread between two summation orders
+    W1 = U.copy()
+    O.Lattice(dims, nth=1).coulomb(W1, 0.09, 1e-14, 650, facg=True)
     tr = lambda X: X[:, :3, 0] + X[:, :3, 8] + X[:, :3, 16]
-    assert np.abs(tr(Wg) - tr(W)).max() < 1e-10
+    spread = np.abs(tr(W1) - tr(W)).max()
+    assert np.abs(tr(Wg) - tr(W)).max() < max(1e-10, 20 * spread), spread
 
 
 def test_heatbath_batched_slices():
