@@ -21,8 +21,18 @@
  * does), and partials are added in thread order.  With nth equal to GLU's
  * OMP_NUM_THREADS the summation order is GLU's.
  *
- * Parity status of each part is recorded in DESIGN.md and in the header of
- * tests/test_oracle_pins.py.
+ * Each function is pinned by a check that a plausible mistake would fail
+ * (tests/test_oracle_pins.py lists them): the NERSC reader, plaquette and
+ * link trace by the CONF/lat..400 header; p^2 by gen_p_sq(0) == 1 and its
+ * closed form; the FFT by a definition-level DFT and closed forms; the
+ * derivative by the generic Hermitian projection, theta_lin == 2 theta_FA,
+ * sum_x Delta(x) == 0 and an abelian closed form; reunitarisation,
+ * set_gauge_matrix, gtransform_local and the triple trace by unitarity and
+ * matrix products; the line search by the functional after trial updates;
+ * the loops by pure-gauge inputs and invariants.  Parity unpinned: the
+ * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
+ * trajectories, and or_heatbath, which is this build's own generator (a
+ * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.
  */
 #ifndef GLU_ORACLE_H
 #define GLU_ORACLE_H
