@@ -31,7 +31,7 @@
  * matrix products; the line search by the functional after trial updates;
  * the loops by pure-gauge inputs and invariants.  Parity unpinned: the
  * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
- * trajectories, and or_heatbath, which is this build's own generator (a
+ * trajectories, and or_heatbath, which is an independent generator (a
  * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.
  */
 #ifndef GLU_ORACLE_H
