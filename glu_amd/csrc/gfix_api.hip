@@ -408,16 +408,19 @@ template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
   launch_fin_sd(c->part, c->nblk, c->ctrl, s);
   enq_update_deriv<T>(c);
+  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
 }
 
-// one FACG iteration (Gfix/FACG.c:272-334)
+// one FACG iteration (Gfix/FACG.c:272-334).  On entry with a pending
+// derivative, theta of the current links is already in the control block
+// (set with the end of the previous iteration or the SD start).
 template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
   if (!c->deriv_ready[tix<T>()]) {
     GFIX_PROF(c, pk<T>(PK_DERIV), launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, s));
+    launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
     c->deriv_ready[tix<T>()] = 1;
   }
-  launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
   launch_fin_pr(c->part, c->nblk, c->ctrl, s);
@@ -425,7 +428,7 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, geo_order(c, c->ord_ls), c->ctrl, 0, s));
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
   enq_update_deriv<T>(c);
-  launch_iter_end(c->ctrl, c->trace, 0, s);
+  launch_fin_iter(c->part, c->nblk, c->ctrl, c->trace, landau_norm(c), s);
 }
 
 // one FASD iteration (Gfix/FACG.c:202-235, fixed step GF_TUNE)
@@ -528,6 +531,7 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
   if (c->run.method == GLU_GFIX_CG && !c->deriv_ready[tix<T>()]) {
     const Bufs b = bufs_for<T>(c);
     launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, c->stream);
+    launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), c->stream);
     c->deriv_ready[tix<T>()] = 1;
   }
   const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1) + 4 * c->upar[tix<T>()];
@@ -616,7 +620,10 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     if (int rc = enq_batch(c)) return rc;
     return 2;
   }
-  // a segment ended
+  // a segment ended; the iteration that stopped it did not finalise the
+  // theta of the derivative it left, so a CG segment that resumes from
+  // here takes the derivative again
+  c->deriv_ready[0] = c->deriv_ready[1] = 0;
   c->trace_n = h.trace_n;
   const double theta = h.theta;
   const long long loc = h.iters;

@@ -72,6 +72,7 @@ void launch_fin_sd(const double *part, long long nb, Ctrl *ctrl, hipStream_t s);
 void launch_fin_ls(const double *part, long long nb, Ctrl *ctrl, double *trace, int force, double pl1, double pl2,
                    hipStream_t s);
 void launch_iter_end(Ctrl *ctrl, double *trace, int fasd, hipStream_t s);
+void launch_fin_iter(const double *part, long long nb, Ctrl *ctrl, double *trace, double norm, hipStream_t s);
 void launch_begin_segment(Ctrl *ctrl, int sd, hipStream_t s);
 void launch_demote(const void *U, void *W, long long n, hipStream_t s);
 void launch_apply_gauge(void *U, const void *G, const Geo &g, hipStream_t s);

@@ -394,30 +394,51 @@ __global__ void k_fin_ls(const double *__restrict__ part, long long nb, Ctrl *ct
   }
 }
 
-// end of a loop iteration: count, record, stop test (Gfix/FACG.c:331-334)
-__global__ void k_iter_end(Ctrl *ctrl, double *trace, int fasd) {
-  if (ctrl->done) return;
+// end of a loop iteration: count, record, stop test (Gfix/FACG.c:331-334);
+// returns the done flag
+__device__ int iter_end_body(Ctrl *ctrl, double *trace, int fasd) {
   if (fasd) {
     ctrl->alpha = ctrl->gf_alpha;
     ctrl->beta = 0.0;
   }
-  ctrl->iters++;
+  const double theta = ctrl->theta;
+  const long long iters = ctrl->iters + 1;
+  ctrl->iters = iters;
   ctrl->total++;
   if (trace && ctrl->trace_n < ctrl->trace_cap) {
     double *r = trace + 3 * ctrl->trace_n;
-    r[0] = ctrl->theta;
+    r[0] = theta;
     r[1] = ctrl->alpha;
     r[2] = ctrl->beta;
     ctrl->trace_n++;
   }
   ctrl->stale_insum = ctrl->insum;
-  ctrl->stale_theta = ctrl->theta;
+  ctrl->stale_theta = theta;
   for (int k = 0; k < 3; k++) ctrl->stale_probe[k] = ctrl->did_ls ? ctrl->probe[k] : 0.0;
-  if (!(ctrl->theta > ctrl->acc && ctrl->iters < ctrl->max_iters)) ctrl->done = 1;
-  if (ctrl->switch_theta > 0.0 && ctrl->theta < ctrl->switch_theta) {
+  int done = !(theta > ctrl->acc && iters < ctrl->max_iters);
+  if (ctrl->switch_theta > 0.0 && theta < ctrl->switch_theta) {
     ctrl->switched = 1;
-    ctrl->done = 1;
+    done = 1;
   }
+  ctrl->done = done;
+  return done;
+}
+
+__global__ void k_iter_end(Ctrl *ctrl, double *trace, int fasd) {
+  if (ctrl->done) return;
+  iter_end_body(ctrl, trace, fasd);
+}
+
+// The end of a CG iteration and the theta of the next one in one launch:
+// the stop test on theta_k, then, unless it stopped, theta_k+1 from the
+// partials of the fused update + derivative (as k_fin_theta; the SD start
+// is over, so no stale slots enter).  A stopped run keeps the theta it
+// tested, and the derivative it left is taken again if a segment resumes.
+__global__ void k_fin_iter(const double *__restrict__ part, long long nb, Ctrl *ctrl, double *trace, double norm) {
+  if (ctrl->done) return;
+  double s[1];
+  final_sum<1>(part, nb, s);
+  if (threadIdx.x == 0 && !iter_end_body(ctrl, trace, 0)) ctrl->theta = s[0] * norm;
 }
 
 __global__ void k_begin_segment(Ctrl *ctrl, int sd) {
@@ -652,6 +673,9 @@ void launch_fin_sd(const double *part, long long nb, Ctrl *ctrl, hipStream_t s) 
 void launch_fin_ls(const double *part, long long nb, Ctrl *ctrl, double *trace, int force, double pl1, double pl2,
                    hipStream_t s) {
   hipLaunchKernelGGL(k_fin_ls, dim3(1), dim3(1024), 0, s, part, nb, ctrl, trace, force, pl1, pl2);
+}
+void launch_fin_iter(const double *part, long long nb, Ctrl *ctrl, double *trace, double norm, hipStream_t s) {
+  hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(1024), 0, s, part, nb, ctrl, trace, norm);
 }
 void launch_iter_end(Ctrl *ctrl, double *trace, int fasd, hipStream_t s) {
   hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctrl, trace, fasd);
