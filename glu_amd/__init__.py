@@ -34,7 +34,7 @@ EXPORTS = [
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
     "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
-    "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy",
+    "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv"]
@@ -65,6 +65,7 @@ def lib():
             "glu_gfix_measure": (C.c_int, [P, DP, DP, DP, DP, DP]),
             "glu_gfix_theta": (C.c_int, [P, DP]),
             "glu_gfix_reunit": (C.c_int, [P]),
+            "glu_gfix_write_nersc": (C.c_int, [P, C.c_char_p, C.c_int, C.c_int, C.c_char_p, SZ]),
             "glu_gfix_mixed_split": (C.c_int, [P, C.POINTER(SZ), C.POINTER(SZ)]),
             "glu_gfix_slice_info": (C.c_int, [P, C.POINTER(SZ), DP]),
             "glu_gfix_stage_deriv": (C.c_int, [P, DP, DP]),
@@ -210,6 +211,13 @@ class Gfix:
 
     def reunit(self):
         _check(lib().glu_gfix_reunit(self._h), "reunit")
+
+    def write_nersc(self, path: str, two_rows: bool = False, big_endian: bool = True, label: str = "",
+                    sequence: int = 1):
+        """GLU's write_lat, NERSC branch: reunitarises the links in place,
+        then writes them with the header's checksum, plaquette and trace."""
+        _check(lib().glu_gfix_write_nersc(self._h, os.fspath(path).encode(), int(two_rows), int(big_endian),
+                                          label.encode(), sequence), "write_nersc")
 
     def set_switch_theta(self, theta: float):
         _check(lib().glu_gfix_set_switch_theta(self._h, theta), "set_switch_theta")

@@ -226,9 +226,7 @@ extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
 // ---------------------------------------------------------------------------
 // links in / out
 // ---------------------------------------------------------------------------
-static const long long kChunk = 1 << 17;  // sites per staging transfer
-
-static int ensure_stage(glu_gfix_ctx *c) {
+int ensure_stage(glu_gfix_ctx *c) {
   if (c->stage) return 0;
   if (hipMalloc(&c->stage, (size_t)kChunk * 72 * sizeof(double)) != hipSuccess)
     return gfix_fail(-2, "staging allocation failed");

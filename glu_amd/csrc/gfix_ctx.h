@@ -120,3 +120,9 @@ int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *it
 gfix::Geo geo_order(const glu_gfix_ctx *c, int order);
 extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed);
 extern "C" int glu_gfix_measure(glu_gfix_ctx *c, double *plaq, double *tr, double *mx, double *func, double *tlink);
+extern "C" int glu_gfix_reunit(glu_gfix_ctx *c);
+
+// links cross the host boundary in chunks of this many sites through the
+// staging buffer (kChunk * 72 doubles, allocated on first use)
+constexpr long long kChunk = 1 << 17;
+int ensure_stage(glu_gfix_ctx *c);

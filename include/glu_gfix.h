@@ -170,6 +170,16 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 /* Reunitarise every device link (Geometry/random_config.c:43-56). */
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
 
+/* Write the device links as a NERSC file: the NERSC branch of write_lat
+   (IO/writers.c:330-422) with write_header_NERSC (IO/write_headers.c:77-138).
+   As in GLU the links are reunitarised in place first.  two_rows selects
+   4D_SU3_GAUGE (rows 0 and 1 of each link) over 4D_SU3_GAUGE_3x3;
+   big_endian IEEE64BIG (GLU's default) over IEEE64LITTLE; label and
+   sequence fill ENSEMBLE_LABEL and SEQUENCE_NUMBER (GLU's reader refuses
+   0).  Returns -3 when the file cannot be written. */
+int glu_gfix_write_nersc(glu_gfix_ctx *ctx, const char *path, int two_rows, int big_endian,
+                         const char *label, size_t sequence);
+
 /* Mixed precision: run FP32 until theta < switch_theta, then FP64 (default
    1e-7, or the GLU_GFIX_SWITCH_THETA environment variable at creation).
    The FP32 phase caps the Polak-Ribiere beta at 3 (GLU_GFIX_BETA_CAP;

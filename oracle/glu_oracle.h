@@ -145,8 +145,9 @@ typedef struct {
   long data_offset;
 } or_nersc_info;
 int or_read_nersc(const char *path, or_nersc_info *info, double *U);
-int or_write_nersc(const char *path, const or_lat *L, const double *U,
-                   int two_rows);
+int or_write_nersc(const char *path, const or_lat *L, double *U, int two_rows,
+                   int big_endian, int reunit, const char *label,
+                   size_t sequence, const char *creator, const char *date);
 
 /* ---------------- input generation (not on the hot path) ---------------- */
 /* quenched Wilson heatbath + overrelaxation, Cabibbo-Marinari with

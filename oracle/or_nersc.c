@@ -125,12 +125,22 @@ int or_read_nersc(const char *path, or_nersc_info *info, double *U)
   return 0;
 }
 
-/* big-endian IEEE64 NERSC writer (GLU's default output endianness,
-   Headers/GLU_definitions.h:651-662; header keys IO/write_headers.c:77-138) */
-int or_write_nersc(const char *path, const or_lat *L, const double *U,
-                   int two_rows)
+/* NERSC writer, the NERSC branch of write_lat (IO/writers.c:330-422): the
+   links are reunitarised in place first (:333, skipped with reunit = 0);
+   the checksum is the uint32 sum of the 32-bit halves of every stored
+   double in host order (:49-69, :183-203); the header keys follow
+   write_header_NERSC (IO/write_headers.c:77-138); the data are rows 0 and 1
+   of each link (4D_SU3_GAUGE, LOOP_GAUGE) or all nine entries (3x3), in
+   IEEE64 of the chosen endianness (GLU's default is big,
+   Headers/GLU_definitions.h:651-662).  creator and date stand for $USER
+   and the UTC asctime() GLU prints. */
+int or_write_nersc(const char *path, const or_lat *L, double *U, int two_rows,
+                   int big_endian, int reunit, const char *label,
+                   size_t sequence, const char *creator, const char *date)
 {
+  if (reunit) or_latt_reunit(L, U);
   const size_t per = two_rows ? 12 : 18, nreal = L->V * OR_ND * per;
+  const int swap = big_endian != (__BYTE_ORDER__ == __ORDER_BIG_ENDIAN__);
   uint64_t *buf = malloc(nreal * 8);
   uint32_t sum = 0;
   for (size_t l = 0; l < L->V * OR_ND; l++)
@@ -138,7 +148,7 @@ int or_write_nersc(const char *path, const or_lat *L, const double *U,
       uint64_t w;
       memcpy(&w, U + l * 18 + e, 8);
       sum += (uint32_t)(w & 0xffffffffu) + (uint32_t)(w >> 32);
-      buf[l * per + e] = __BYTE_ORDER__ == __ORDER_BIG_ENDIAN__ ? w : bswap64(w);
+      buf[l * per + e] = swap ? bswap64(w) : w;
     }
   FILE *f = fopen(path, "wb");
   if (!f) { free(buf); return -1; }
@@ -148,11 +158,16 @@ int or_write_nersc(const char *path, const or_lat *L, const double *U,
           two_rows ? "4D_SU3_GAUGE" : "4D_SU3_GAUGE_3x3");
   for (int mu = 0; mu < OR_ND; mu++)
     fprintf(f, "DIMENSION_%d = %zu\n", mu + 1, L->dims[mu]);
-  fprintf(f, "LINK_TRACE = %1.15e\nPLAQUETTE = %1.15e\n",
-          or_links(L, U, &tmax), or_plaquette(L, U));
+  fprintf(f, "CHECKSUM = %x\n", sum);
+  fprintf(f, "LINK_TRACE = %1.15f\n", or_links(L, U, &tmax));
+  fprintf(f, "PLAQUETTE = %1.15f\n", or_plaquette(L, U));
   for (int mu = 0; mu < OR_ND; mu++) fprintf(f, "BOUNDARY_%d = PERIODIC\n", mu + 1);
-  fprintf(f, "CHECKSUM = %x\nSEQUENCE_NUMBER = 1\n"
-             "FLOATING_POINT = IEEE64BIG\nEND_HEADER\n", sum);
+  fprintf(f, "ENSEMBLE_ID = ukqcd\nSEQUENCE_NUMBER = %zu\n"
+             "ENSEMBLE_LABEL = %s\nCREATOR_MACHINE = %s\n"
+             "CREATION_DATE = %s\nFLOATING_POINT = %s\nEND_HEADER\n",
+          sequence, label ? label : "", creator ? creator : "(null)",
+          date ? date : "Mon Jan  0 00:00:00 1970",
+          big_endian ? "IEEE64BIG" : "IEEE64LITTLE");
   const size_t wrote = fwrite(buf, 8, nreal, f);
   fclose(f);
   free(buf);

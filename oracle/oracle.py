@@ -93,7 +93,8 @@ def lib():
             "or_random_gtrans": (None, [LP, C.c_void_p, DP]),
             "or_gtransform": (None, [LP, DP, DP]),
             "or_read_nersc": (C.c_int, [C.c_char_p, C.POINTER(NerscInfo), DP]),
-            "or_write_nersc": (C.c_int, [C.c_char_p, LP, DP, C.c_int]),
+            "or_write_nersc": (C.c_int, [C.c_char_p, LP, DP, C.c_int, C.c_int, C.c_int, C.c_char_p, SZ,
+                                         C.c_char_p, C.c_char_p]),
             "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.c_int]),
         }
@@ -255,8 +256,13 @@ def read_nersc(path: str):
     return U, info
 
 
-def write_nersc(path: str, lat: Lattice, U: np.ndarray, two_rows=False):
-    rc = lib().or_write_nersc(path.encode(), lat.ref, _dp(U), int(two_rows))
+def write_nersc(path: str, lat: Lattice, U: np.ndarray, two_rows=False, big_endian=True, reunit=True,
+                label="", sequence=1, creator="oracle", date="Mon Jan  0 00:00:00 1970"):
+    """GLU's write_lat, NERSC branch; with reunit (GLU always does) U is
+    reunitarised in place, as GLU does to the lattice it writes."""
+    assert U.flags.c_contiguous and U.dtype == np.float64
+    rc = lib().or_write_nersc(path.encode(), lat.ref, _dp(U), int(two_rows), int(big_endian), int(reunit),
+                              label.encode(), sequence, creator.encode(), date.encode())
     if rc != 0:
         raise IOError(f"NERSC write failed ({rc})")
 

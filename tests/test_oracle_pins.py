@@ -84,10 +84,65 @@ def test_nersc_known_answers(conf):
 def test_nersc_roundtrip(tmp_path, conf):
     U, info, L = conf
     p = str(tmp_path / "w.nersc")
-    O.write_nersc(p, L, U, two_rows=False)
+    W = U.copy()
+    O.write_nersc(p, L, W, two_rows=False, reunit=False)
     U2, info2 = O.read_nersc(p)
     assert np.array_equal(U, U2)
     assert info2.checksum_header == info2.checksum_computed
+
+
+def payload(path, info):
+    with open(path, "rb") as f:
+        f.seek(info.data_offset)
+        return f.read()
+
+
+def test_nersc_writer_reproduces_the_fixture(tmp_path, conf):
+    """Writing the fixture's links back as 4D_SU3_GAUGE, little-endian,
+    without the reunitarisation gives the file's own bytes and its header
+    checksum f2ee7c36 (IO/writers.c:49-69, 183-203)."""
+    U, info, L = conf
+    p = str(tmp_path / "f.nersc")
+    O.write_nersc(p, L, U.copy(), two_rows=True, big_endian=False, reunit=False)
+    _, info2 = O.read_nersc(p)
+    assert info2.checksum_header == 0xF2EE7C36 and info2.two_rows and not info2.big_endian
+    assert payload(p, info2) == payload(CONF, info)
+    # big-endian output: the same doubles, each byte-reversed
+    q = str(tmp_path / "b.nersc")
+    O.write_nersc(q, L, U.copy(), two_rows=True, big_endian=True, reunit=False)
+    _, info3 = O.read_nersc(q)
+    assert info3.big_endian and info3.checksum_header == info3.checksum_computed == 0xF2EE7C36
+    le = np.frombuffer(payload(p, info2), "<f8")
+    assert np.array_equal(np.frombuffer(payload(q, info3), ">f8"), le)
+
+
+def test_nersc_writer_header_after_reunitarisation(tmp_path, conf):
+    """GLU reunitarises the lattice before writing (IO/writers.c:333) and
+    prints the plaquette and link trace of what it writes, %1.15f, in
+    write_header_NERSC's key order (IO/write_headers.c:77-138)."""
+    U, info, L = conf
+    p = str(tmp_path / "r.nersc")
+    W = U.copy()
+    O.write_nersc(p, L, W, two_rows=False, big_endian=True, label="fixed", sequence=400)
+    R = U.copy()
+    L.reunit(R)
+    assert np.array_equal(W, R)  # in place, as GLU does to its lattice
+    U2, info2 = O.read_nersc(p)
+    assert np.array_equal(U2, R)
+    assert info2.checksum_header == info2.checksum_computed
+    with open(p, "rb") as f:
+        head = f.read(info2.data_offset).decode().splitlines()
+    keys = [h.split(" = ")[0] for h in head]
+    assert keys == ["BEGIN_HEADER", "HDR_VERSION", "DATATYPE", "STORAGE_FORMAT"] + \
+        [f"DIMENSION_{k}" for k in range(1, 5)] + ["CHECKSUM", "LINK_TRACE", "PLAQUETTE"] + \
+        [f"BOUNDARY_{k}" for k in range(1, 5)] + \
+        ["ENSEMBLE_ID", "SEQUENCE_NUMBER", "ENSEMBLE_LABEL", "CREATOR_MACHINE", "CREATION_DATE",
+         "FLOATING_POINT", "END_HEADER"]
+    assert f"PLAQUETTE = {L.plaquette(R):1.15f}" in head
+    assert f"LINK_TRACE = {L.links(R)[0]:1.15f}" in head
+    assert "SEQUENCE_NUMBER = 400" in head and "ENSEMBLE_LABEL = fixed" in head
+    # the reunitarisation moves the links by rounding only
+    assert abs(info2.plaquette - 0.5985455591) < 1e-10
 
 
 # ------------------------------------------------------------- geometry ---
