@@ -23,7 +23,8 @@
  *
  * Each function is pinned by a check that a plausible mistake would fail
  * (tests/test_oracle_pins.py lists them): the NERSC reader, plaquette and
- * link trace by the CONF/lat..400 header; p^2 by gen_p_sq(0) == 1 and its
+ * link trace by the CONF/lat..400 header; the NERSC writer by reproducing
+ * that file's data bytes and checksum from its links; p^2 by gen_p_sq(0) == 1 and its
  * closed form; the FFT by a definition-level DFT and closed forms; the
  * derivative by the generic Hermitian projection, theta_lin == 2 theta_FA,
  * sum_x Delta(x) == 0 and an abelian closed form; reunitarisation,
