@@ -150,6 +150,20 @@ int or_write_nersc(const char *path, const or_lat *L, double *U, int two_rows,
                    int big_endian, int reunit, const char *label,
                    size_t sequence, const char *creator, const char *date);
 
+/* ---------------- gluon propagators (Cuts/, downstream of the path) ------ */
+/* the momentum list of a cut (type 0 hypercubic, 1 psq, 2 cylinder,
+   3 cylinder and cone): *n momenta, mom[n][4] signed, idx[n] their sites in
+   the DFT output; returns -1 if n > cap, -2 for an unknown type */
+int or_momentum_list(const or_lat *L, int type, size_t max_mom,
+                     double cyl_width, double angle, int *mom, size_t *idx,
+                     size_t cap, size_t *n);
+/* transverse G(p^2) and longitudinal F(p^2) over that list */
+int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
+                  double cyl_width, double angle, int *mom, double *trans,
+                  double *longi, size_t cap, size_t *n);
+int or_write_glueprop(const char *path, size_t n, const int *mom,
+                      const double *trans, const double *longi);
+
 /* ---------------- input generation (not on the hot path) ---------------- */
 /* quenched Wilson heatbath + overrelaxation, Cabibbo-Marinari with
    Kennedy-Pendleton SU(2) heatbath; own seeded generator */

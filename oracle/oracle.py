@@ -21,6 +21,7 @@ _LIB = None
 
 SZ = C.c_size_t
 DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
 
 
 class Lat(C.Structure):
@@ -95,6 +96,11 @@ def lib():
             "or_read_nersc": (C.c_int, [C.c_char_p, C.POINTER(NerscInfo), DP]),
             "or_write_nersc": (C.c_int, [C.c_char_p, LP, DP, C.c_int, C.c_int, C.c_int, C.c_char_p, SZ,
                                          C.c_char_p, C.c_char_p]),
+            "or_momentum_list": (C.c_int, [LP, C.c_int, SZ, C.c_double, C.c_double, IP,
+                                           C.POINTER(SZ), SZ, C.POINTER(SZ)]),
+            "or_gluon_prop": (C.c_int, [LP, DP, C.c_int, SZ, C.c_double, C.c_double, IP, DP, DP, SZ,
+                                        C.POINTER(SZ)]),
+            "or_write_glueprop": (C.c_int, [C.c_char_p, SZ, IP, DP, DP]),
             "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.c_int]),
         }
@@ -254,6 +260,48 @@ def read_nersc(path: str):
     if rc != 0:
         raise IOError(f"NERSC data read failed ({rc}): {path}")
     return U, info
+
+
+CUTS = {"hypercubic": 0, "psq": 1, "cylinder": 2, "conical": 3}
+
+
+def momentum_list(lat: Lattice, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0):
+    """GLU's momentum list of a cut: (mom[n, 4] signed ints, sites[n])."""
+    n = SZ()
+    t = CUTS[cut] if isinstance(cut, str) else cut
+    rc = lib().or_momentum_list(lat.ref, t, max_mom, cyl_width, angle, None, None, 0, C.byref(n))
+    if rc == -2:
+        raise ValueError(f"unknown cut {cut}")
+    mom = np.zeros((max(n.value, 1), 4), np.int32)
+    idx = np.zeros(max(n.value, 1), np.uintp)
+    rc = lib().or_momentum_list(lat.ref, t, max_mom, cyl_width, angle, mom.ctypes.data_as(IP),
+                                idx.ctypes.data_as(C.POINTER(SZ)), n.value, C.byref(n))
+    assert rc == 0
+    return mom[:n.value], idx[:n.value].astype(np.int64)
+
+
+def gluon_prop(lat: Lattice, U: np.ndarray, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0):
+    """Transverse and longitudinal propagators of GLU's GLUON_PROPS cuts:
+    (mom[n, 4], G[n], F[n])."""
+    mom0, _ = momentum_list(lat, cut, max_mom, cyl_width, angle)
+    n = SZ()
+    cap = max(len(mom0), 1)
+    mom = np.zeros((cap, 4), np.int32)
+    G, F = np.zeros(cap), np.zeros(cap)
+    rc = lib().or_gluon_prop(lat.ref, _dp(np.ascontiguousarray(U)), CUTS[cut] if isinstance(cut, str) else cut,
+                             max_mom, cyl_width, angle, mom.ctypes.data_as(IP), _dp(G), _dp(F), cap,
+                             C.byref(n))
+    assert rc == 0
+    k = n.value
+    return mom[:k], G[:k], F[:k]
+
+
+def write_glueprop(path: str, mom: np.ndarray, G: np.ndarray, F: np.ndarray):
+    mom = np.ascontiguousarray(mom, np.int32)
+    rc = lib().or_write_glueprop(path.encode(), len(G), mom.ctypes.data_as(IP), _dp(np.ascontiguousarray(G)),
+                                 _dp(np.ascontiguousarray(F)))
+    if rc != 0:
+        raise IOError(f"propagator write failed ({rc})")
 
 
 def write_nersc(path: str, lat: Lattice, U: np.ndarray, two_rows=False, big_endian=True, reunit=True,
