@@ -9,10 +9,12 @@ a device heatbath config (beta = 6.0, cold start, 4 overrelaxation passes
 per sweep) with a seeded random gauge transform, generated once before the
 timed region.
 
-Prints one JSON line (see the keys below).  Two further workloads print a
+Prints one JSON line (see the keys below).  Three further workloads print a
 line of their own: --workload coulomb (48^3x96 Coulomb FACG, the 96 time
-slices batched) and --workload serve (a Poisson stream of 64 Landau
-requests of mixed sizes through the serving loop).
+slices batched), --workload serve (a Poisson stream of 64 Landau
+requests of mixed sizes through the serving loop) and --workload glueprop
+(gluon propagators of a Landau-fixed 32^3x64 config, GLU's GLUON_PROPS
+cuts).
 """
 from __future__ import annotations
 
@@ -54,7 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="CPU baseline sample: k and 2k oracle iterations")
-    ap.add_argument("--workload", choices=("landau", "coulomb", "serve"), default="landau")
+    ap.add_argument("--workload", choices=("landau", "coulomb", "serve", "glueprop"), default="landau")
+    ap.add_argument("--max-mom", type=int, default=64, help="glueprop: PSQ cut p^2 <= max_mom")
     ap.add_argument("--graph", action="store_true", help="replay iteration batches from HIP graphs")
     ap.add_argument("--requests", type=int, default=64)
     ap.add_argument("--slots", type=lambda v: [int(x) for x in v.split(",")], default=[4, 8, 1],
@@ -135,6 +138,8 @@ def main():
         return coulomb_main(a)
     if a.workload == "serve":
         return serve_main(a)
+    if a.workload == "glueprop":
+        return glueprop_main(a)
     dims = tuple(int(x) for x in a.dims.split(","))
     V = math.prod(dims)
     print(f"[bench] lattice {dims} V={V}: generating input "
@@ -310,6 +315,86 @@ def coulomb_main(a):
         "slice_iters": {"min": min(sits), "max": max(sits), "mean": round(sum(sits) / len(sits), 1)},
         "max_slice_theta": float(max(sth)), "plaquette_drift": abs(m["plaquette"] - plaq),
         "stream_GBps": round(gbs, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+# gluon propagators, B/site per launch: the Lie-field packing reads the
+# 2-column FP64 links (384) and writes 16 packed complex fields (256); an
+# FFT axis pass reads and writes those 16 fields
+GLUEPROP_ALG = {"lie_pack": 640, "fft_axis": 512}
+
+
+def glueprop_main(a):
+    """Transverse / longitudinal gluon propagators (GLU's GLUON_PROPS cuts,
+    PSQ cut) of a Landau-fixed config: one step = one measurement on the
+    device links (Lie fields, 16-field transform, Landau phase,
+    projections over the momentum list, copy of the results to the host)."""
+    import numpy as np
+
+    import glu_amd as gl
+    from oracle import oracle as O
+
+    dims = tuple(int(x) for x in a.dims.split(","))
+    V = math.prod(dims)
+    g = gl.Gfix(dims, gl.LANDAU, gl.MIXED)
+    t = time.perf_counter()
+    g.heatbath(6.0, a.sweeps, 4, seed=a.seed, cold_start=True)
+    g.random_transform(seed=a.seed + 1)
+    g.autotune()
+    g.set_switch_theta(a.switch_theta)
+    it, th, _ = g.landau(gl.CG, 0.08, 1e-14, 1000, restart_seed=a.seed + 7)
+    print(f"[bench] glueprop {dims}: input fixed in {time.perf_counter() - t:.1f}s ({it} iterations, "
+          f"theta {th:.2e})", file=sys.stderr, flush=True)
+    for _ in range(max(a.warmup, 1)):
+        mom, G, F, err = g.gluon_props("psq", a.max_mom)
+    g.profile(sum(kernel_mask(k, "fp64") for k in ("lie_pack", "fft_axis", "gluon_prop")))
+    g.gluon_props("psq", a.max_mom)
+    prof = g.profile_read()
+    g.profile(0)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        mom, G, F, err = g.gluon_props("psq", a.max_mom)
+        lat.append(time.perf_counter() - t)
+    total = time.perf_counter() - t0
+    U = g.get_links()
+    g.close()
+    kern = {}
+    for (name, prec), (n, mean, tot) in prof.items():
+        kern[name] = {"launches": n, "mean_ms": round(mean, 4), "total_ms": round(tot, 3)}
+        if name in GLUEPROP_ALG:
+            kern[name]["GBps"] = round(GLUEPROP_ALG[name] * V / (mean * 1e-3) / 1e9, 1)
+    dom = max(GLUEPROP_ALG, key=lambda k: kern.get(k, {}).get("total_ms", 0.0))
+    ach = GLUEPROP_ALG[dom] * V / (kern[dom]["mean_ms"] * 1e-3) / 1e9
+    # CPU: the oracle (GLU's 36-field transform, one thread) on a 16^3x32
+    # sample of the same fixed field's shape, scaled by the volume
+    cpu = None
+    if not a.no_cpu:
+        sd = (16, 16, 16, 32)
+        Ls = O.Lattice(sd, nth=1)
+        Us = np.ascontiguousarray(U.reshape(dims[::-1] + (4, 18))[:sd[3], :sd[2], :sd[1], :sd[0]]).reshape(-1, 4, 18)
+        t = time.perf_counter()
+        O.gluon_prop(Ls, Us, "psq", a.max_mom)
+        ts = time.perf_counter() - t
+        scale = V / math.prod(sd)
+        cpu = {"value": round(1.0 / (ts * scale), 5), "unit": "measurements/s", "cores": 1, "kind": "port",
+               "sample": f"oracle on a 16x16x16x32 corner of the same field ({ts:.2f}s), scaled by the volume "
+                         f"ratio {scale:.0f} (FFT log factor ignored)"}
+    line = {
+        "metric": "gluon propagator measurements/s at 32^3x64 (GLU GLUON_PROPS, PSQ cut)",
+        "value": round(a.steps / total, 3), "unit": "measurements/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, seed {a.seed}), "
+                 f"random gauge transform, Landau-fixed to theta<1e-14 before the timed region"),
+        "config": {"workload": "gluon_props_psq_" + "x".join(map(str, dims)), "lattice": list(dims),
+                   "cut": "psq", "max_mom": a.max_mom, "momenta": int(len(G))},
+        "p50_ms": round(1e3 * pct(lat, 0.5), 3), "pspace_landau_error": err,
+        "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None},
+        "kernels": kern, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 

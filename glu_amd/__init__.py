@@ -35,9 +35,12 @@ EXPORTS = [
     "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
+    "glu_gfix_gluon_props",
 ]
 
-KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv"]
+KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
+           "gluon_prop"]
+CUTS = {"hypercubic": 0, "psq": 1, "cylinder": 2, "conical": 3}
 
 
 class GfixError(RuntimeError):
@@ -66,6 +69,8 @@ def lib():
             "glu_gfix_theta": (C.c_int, [P, DP]),
             "glu_gfix_reunit": (C.c_int, [P]),
             "glu_gfix_write_nersc": (C.c_int, [P, C.c_char_p, C.c_int, C.c_int, C.c_char_p, SZ]),
+            "glu_gfix_gluon_props": (C.c_int, [P, C.c_int, SZ, C.c_double, C.c_double, C.c_char_p,
+                                               C.POINTER(SZ), C.POINTER(C.c_int), DP, DP, SZ, DP]),
             "glu_gfix_mixed_split": (C.c_int, [P, C.POINTER(SZ), C.POINTER(SZ)]),
             "glu_gfix_slice_info": (C.c_int, [P, C.POINTER(SZ), DP]),
             "glu_gfix_stage_deriv": (C.c_int, [P, DP, DP]),
@@ -212,6 +217,22 @@ class Gfix:
     def reunit(self):
         _check(lib().glu_gfix_reunit(self._h), "reunit")
 
+    def gluon_props(self, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0, path=None):
+        """GLU's GLUON_PROPS cuts on the device links: (mom[n, 4], G[n], F[n],
+        momentum-space Landau-condition figure)."""
+        t = CUTS[cut] if isinstance(cut, str) else int(cut)
+        n = SZ()
+        _check(lib().glu_gfix_gluon_props(self._h, t, max_mom, cyl_width, angle, None, C.byref(n), None, None,
+                                          None, 0, None), "gluon_props")
+        k = n.value
+        mom = np.zeros((k, 4), np.int32)
+        G, F, err = np.zeros(k), np.zeros(k), C.c_double()
+        _check(lib().glu_gfix_gluon_props(self._h, t, max_mom, cyl_width, angle,
+                                          None if path is None else os.fspath(path).encode(), C.byref(n),
+                                          mom.ctypes.data_as(C.POINTER(C.c_int)), _dp(G), _dp(F), k,
+                                          C.byref(err)), "gluon_props")
+        return mom, G, F, err.value
+
     def write_nersc(self, path: str, two_rows: bool = False, big_endian: bool = True, label: str = "",
                     sequence: int = 1):
         """GLU's write_lat, NERSC branch: reunitarises the links in place,
@@ -242,8 +263,8 @@ class Gfix:
         _check(lib().glu_gfix_random_transform(self._h, seed), "random_transform")
 
     def profile(self, mask=-1):
-        """Time kernel classes with HIP events (bit k = KERNELS[k % 7], +7
-        for FP32); -1 all, 0 off."""
+        """Time kernel classes with HIP events (bit k = KERNELS[k], bit
+        k + len(KERNELS) its FP32 variant); -1 all, 0 off."""
         _check(lib().glu_gfix_profile(self._h, int(mask)), "profile")
 
     def profile_read(self):

@@ -208,10 +208,11 @@ extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
   void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
-                  c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->F32, c->sn32, c->old32};
+                  c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->lie, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
     if (b) hipFree(b);
   coulomb_free(c);
+  glueprop_free(c);
   prof_harvest(c);
   for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
   for (hipGraphExec_t &gx : c->graph)

@@ -10,7 +10,7 @@
 struct CoulombState;
 
 // per-kernel event timing (glu_gfix_profile): kernel classes
-enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_EGDERIV, PK_COUNT };
+enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_EGDERIV, PK_LIE, PK_PROP, PK_COUNT };
 struct ProfPending {
   int id;
   hipEvent_t a, b;
@@ -80,6 +80,8 @@ struct glu_gfix_ctx {
   gfix::FftAxis ax[4]{};
   gfix::PsqInfo ps{};
   void *stage = nullptr;  // upload / download staging
+  void *lie = nullptr;    // 16 packed Lie fields of the gluon propagators (first use)
+  void *glue = nullptr;   // device copy of the last momentum list measured
   size_t last_fp32_iters = 0, last_fp64_iters = 0;
   CoulombState *cs = nullptr;
   // profiling: id = kernel class + PK_COUNT for the FP32 variants; prof is
@@ -126,3 +128,4 @@ extern "C" int glu_gfix_reunit(glu_gfix_ctx *c);
 // staging buffer (kChunk * 72 doubles, allocated on first use)
 constexpr long long kChunk = 1 << 17;
 int ensure_stage(glu_gfix_ctx *c);
+void glueprop_free(glu_gfix_ctx *c);

@@ -170,6 +170,25 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 /* Reunitarise every device link (Geometry/random_config.c:43-56). */
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
 
+/* Transverse G(p^2) and longitudinal F(p^2) gluon propagators of the device
+   links: GLU's cuts_struct with dir = GLUON_PROPS (Cuts/cuts.c:114-184),
+   the LINEAR field definition (Hermitian_proj, Matrix_Ops/exactQ.c:358-373),
+   correct_pspace_landau (Cuts/pspace_landau.c:28-86) and
+   compute_gluon_prop (Cuts/glueprop.c:95-151).  The momenta are those of
+   GLU's momentum list for the cut (Cuts/cut_routines.c:300-455; cut_width
+   in lattice units, angle in degrees for the cone), enumerated in GLU's
+   order; *nmom receives their number (with every output NULL, the call
+   only counts).  mom (nmom x 4 signed ints),
+   transverse and longitudinal (nmom doubles each) may be NULL, else hold
+   cap >= nmom entries; pspace_err (or NULL) receives GLU's momentum-space
+   Landau-condition figure.  path (or NULL) receives GLU's big-endian
+   output file (write_mom_veclist + write_g2g3_to_list,
+   Cuts/cut_output.c:311-400).  Landau contexts only. */
+enum { GLU_GFIX_CUT_HYPERCUBIC = 0, GLU_GFIX_CUT_PSQ = 1, GLU_GFIX_CUT_CYLINDER = 2, GLU_GFIX_CUT_CONICAL = 3 };
+int glu_gfix_gluon_props(glu_gfix_ctx *ctx, int cut, size_t max_mom, double cyl_width, double angle,
+                         const char *path, size_t *nmom, int *mom, double *transverse,
+                         double *longitudinal, size_t cap, double *pspace_err);
+
 /* Write the device links as a NERSC file: the NERSC branch of write_lat
    (IO/writers.c:330-422) with write_header_NERSC (IO/write_headers.c:77-138).
    As in GLU the links are reunitarised in place first.  two_rows selects
@@ -223,7 +242,8 @@ int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
 /* Per-kernel timing with HIP events on the context's stream.  Kernel
    classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
    update, 4 line search, 5 gauge update (FASD), 6 gauge update fused with
-   the next derivative (FACG); +7 for the FP32 variants.
+   the next derivative (FACG), 7 Lie-field packing and 8 momentum
+   projection of the gluon propagators; +9 for the FP32 variants.
    `mask` selects the classes timed (bit k = class k; -1 all, 0 off).
    profile_read reports the launches that did work (launches issued past
    convergence return at once and are left out), their mean and total. */
