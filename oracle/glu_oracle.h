@@ -30,7 +30,9 @@
  * sum_x Delta(x) == 0 and an abelian closed form; reunitarisation,
  * set_gauge_matrix, gtransform_local and the triple trace by unitarity and
  * matrix products; the line search by the functional after trial updates;
- * the loops by pure-gauge inputs and invariants.  Parity unpinned: the
+ * the loops by pure-gauge inputs and invariants; the gluon propagators
+ * (tests/test_oracle_glueprop.py) by brute-force momentum lists, numpy and
+ * a plane wave.  Parity unpinned: the
  * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
  * trajectories, and or_heatbath, which is an independent generator (a
  * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.
