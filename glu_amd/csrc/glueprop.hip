@@ -63,8 +63,11 @@ __device__ __forceinline__ double2 conjz(double2 a) { return make_double2(a.x, -
 // correct_pspace_landau (Cuts/pspace_landau.c:28-86), and
 // project_trans_long (Cuts/glueprop.c:60-90) with its normalisation.
 // sin2 / cosk hold 2 sin(k pi / L) and cos(k pi / L) per axis for
-// k = -L/2 .. L - 1 - L/2 (offset L/2, axis mu at off[mu]).  err receives
-// the momentum-space gauge condition term |sum_mu,j p_mu A_mu,j(p)| / NC.
+// k = -L/2 .. L - 1 - L/2 (offset L/2, axis mu at off[mu]), looked up with
+// the listed momentum (GLU's psq_calc recovers it from the site, the same
+// for even extents; for an odd extent it misplaces k = (L-1)/2).  err
+// receives the momentum-space gauge condition term
+// |sum_mu,j p_mu A_mu,j(p)| / NC.
 __global__ __launch_bounds__(kBlock) void k_gluon_prop(const double2 *__restrict__ Z, long long V, int n,
                                                       const int *__restrict__ mom,
                                                       const long long *__restrict__ site,

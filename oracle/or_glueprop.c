@@ -195,13 +195,14 @@ int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
   const size_t zero_pos = nm >> 1;
   for (size_t i = 0; i <= zero_pos && i < nm; i++) {
     const size_t posit = idx[i];
-    /* psq_calc: the momentum of the FFT site in (-L/2 .. L/2) */
+    /* psq_calc: the momentum of the list entry.  GLU recovers it from the
+       site through get_site_2piBZ (Geometry/geometry.c:42-61), which is
+       the same map for even extents; for an odd extent L it sends the site
+       of k = (L-1)/2 through a negative coordinate and the projector of a
+       neighbouring momentum is used.  This uses the listed momentum. */
     double p[OR_ND], tspsq = 0.;
-    size_t s = posit;
     for (int mu = 0; mu < OR_ND; mu++) {
-      const size_t c = s % L->dims[mu];
-      s /= L->dims[mu];
-      const int k = c >= (L->dims[mu] >> 1) ? (int)c - (int)L->dims[mu] : (int)c;
+      const int k = mom[i * OR_ND + mu];
       p[mu] = 2.0 * sin(k * tw[mu] * 0.5);
       tspsq += p[mu] * p[mu];
     }

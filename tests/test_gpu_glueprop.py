@@ -21,6 +21,20 @@ pytestmark = pytest.mark.gpu
 gl = pytest.importorskip("glu_amd")
 
 
+def haar_links(dims, seed):
+    """independent Haar-random SU(3) links in GLU's layout"""
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(dims)) * 4
+    z = rng.standard_normal((n, 3, 3)) + 1j * rng.standard_normal((n, 3, 3))
+    q, r = np.linalg.qr(z)
+    d = np.diagonal(r, axis1=1, axis2=2)
+    q = q * (d / np.abs(d))[:, None, :]
+    q = q / (np.linalg.det(q) ** (1 / 3))[:, None, None]
+    out = np.empty((n, 18))
+    out[:, 0::2], out[:, 1::2] = q.reshape(n, 9).real, q.reshape(n, 9).imag
+    return out.reshape(-1, 4, 18)
+
+
 @pytest.fixture(scope="module")
 def fixed():
     dims = (8, 8, 8, 16)
@@ -54,15 +68,18 @@ def test_gluon_props_match_the_oracle(fixed, cut, max_mom):
     assert err < 1e-5 * err0, (err, err0)
 
 
-def test_gluon_props_of_an_unfixed_field():
+@pytest.mark.parametrize("dims,precision", [((6, 6, 8, 8), gl.FP64), ((6, 6, 8, 8), gl.MIXED),
+                                            ((5, 6, 4, 8), gl.FP64)])
+def test_gluon_props_of_an_unfixed_field(dims, precision):
     """Off Landau gauge the longitudinal part is of the size of the
-    transverse one and is compared on its own scale."""
-    dims = (6, 6, 8, 8)
-    U, _ = heatbath_config(dims, sweeps=10)
-    mo, Go, Fo = O.gluon_prop(O.Lattice(dims, nth=2), U, "psq", 8)
-    with gl.Gfix(dims) as g:
+    transverse one and is compared on its own scale; the links are FP64 in
+    every context, and an odd extent goes through the radix-5 transform."""
+    print(f"gluon_props unfixed {dims} precision={precision}", flush=True)
+    U = haar_links(dims, seed=5)
+    mo, Go, Fo = O.gluon_prop(O.Lattice(dims, nth=2), U, "psq", 3)
+    with gl.Gfix(dims, gl.LANDAU, precision) as g:
         g.set_links(U)
-        mg, Gg, Fg, _ = g.gluon_props("psq", 8)
+        mg, Gg, Fg, _ = g.gluon_props("psq", 3)
     assert np.array_equal(mg, mo)
     assert np.abs(Fo).max() > 0.1 * np.abs(Go).max()
     assert np.abs(Gg - Go).max() < 1e-12 * np.abs(Go).max()

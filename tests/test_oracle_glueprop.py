@@ -115,7 +115,7 @@ def numpy_props(U, dims, mom):
 
 
 @pytest.mark.parametrize("dims,cut,max_mom", [((4, 4, 4, 6), "psq", 3), ((6, 4, 4, 8), "cylinder", 3),
-                                              ((6, 4, 4, 8), "hypercubic", 1)])
+                                              ((6, 4, 4, 8), "hypercubic", 1), ((5, 6, 4, 8), "psq", 3)])
 def test_gluon_prop_against_numpy(dims, cut, max_mom):
     rng = np.random.default_rng(3)
     V = int(np.prod(dims))
