@@ -31,6 +31,9 @@ extern "C" const char *glu_gfix_last_error(void) { return g_err.c_str(); }
 namespace {
 
 constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-455
+// mixed precision: lattices up to this many sites stop the FP32 phase at
+// its first stall (theta 30x above its minimum once below 1e-6)
+constexpr long long kStallSites = 1 << 16;
 constexpr long long kBatch = 8;        // iterations enqueued between host checks
 constexpr int kGluFailures = 7;        // GF_GLU_FAILURES, Headers/GLU_definitions.h:400-402
 
@@ -155,6 +158,11 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
       if (const char *e = getenv("GLU_GFIX_SWITCH_THETA")) c->switch_theta = atof(e);
       if (const char *e = getenv("GLU_GFIX_SAFEGUARD")) c->sg_mode = atoi(e);
       if (const char *e = getenv("GLU_GFIX_BETA_CAP")) c->beta_cap = atof(e);
+      // launch-bound lattices: an FP32 iteration costs about what an FP64
+      // one does, so the FP32 phase hands over at its first stall
+      c->stall_factor = V <= kStallSites ? 30.0 : 0.0;
+      if (const char *e = getenv("GLU_GFIX_STALL_FACTOR")) c->stall_factor = atof(e);
+      if (const char *e = getenv("GLU_GFIX_STALL_THETA")) c->stall_theta = atof(e);
     }
     // FFT axes, twiddles and the p^2 cos tables
     const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
@@ -502,6 +510,9 @@ static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, siz
   h.sg_mode = c->sg_mode;
   h.beta_cap = c->beta_cap;
   h.sg_uphill = 0;
+  h.theta_min = 1e300;
+  h.stall_theta = c->stall_theta;
+  h.stall_factor = c->stall_factor;
   h.max_iters = (long long)max_iters;
   h.done = 1;
   r.max_phase = max_iters;
@@ -635,7 +646,8 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     launch_promote(c->old32, c->old, c->g.V * 4, c->stream);
     double thl = 0.0;
     if (int rc = theta_lin<double>(c, c->U64, &thl)) return rc;
-    const int keep = 0.5 * thl < 10.0 * theta;
+    // a stalled FP32 phase left an uphill direction behind: restart CG
+    const int keep = h.switched == 1 && 0.5 * thl < 10.0 * theta;
     const size_t left = r.max_iters > c->last_fp32_iters ? r.max_iters - c->last_fp32_iters : 1;
     r.phase = PH_FP64;
     if (int rc = phase_ctrl_init(c, keep, 0.0, left)) return rc;

@@ -57,6 +57,9 @@ struct glu_gfix_ctx {
   // FP64 runs keep GLU's beta
   int sg_mode = 2;
   double beta_cap = 3.0;
+  // FP32-phase stall exit (Ctrl::stall_factor, 0: off); set at creation
+  // from the lattice size
+  double stall_theta = 1e-6, stall_factor = 0.0;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner

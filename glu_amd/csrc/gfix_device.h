@@ -119,7 +119,7 @@ struct Ctrl {
   int done;          // set after the update of the iteration that stopped
   int did_ls;        // current iteration used the line search
   int sd_start;      // currently inside the SD start
-  int switched;      // mixed precision reached switch_theta
+  int switched;      // mixed precision: 1 reached switch_theta, 2 stalled
   long long trace_cap;
   long long trace_n;
   // safeguards of the FP32 phase of a mixed run (active while
@@ -129,6 +129,12 @@ struct Ctrl {
   int sg_mode;
   int sg_uphill;  // the last line search found the direction uphill
   double beta_cap;
+  // FP32-phase stall exit: once theta_min < stall_theta, a theta above
+  // stall_factor * theta_min ends the phase early (switched), as at the
+  // switch point; stall_factor 0 disables it
+  double theta_min;
+  double stall_theta;
+  double stall_factor;
 };
 
 // ---------------------------------------------------------------- reduce --

@@ -416,9 +416,14 @@ __device__ int iter_end_body(Ctrl *ctrl, double *trace, int fasd) {
   ctrl->stale_theta = theta;
   for (int k = 0; k < 3; k++) ctrl->stale_probe[k] = ctrl->did_ls ? ctrl->probe[k] : 0.0;
   int done = !(theta > ctrl->acc && iters < ctrl->max_iters);
-  if (ctrl->switch_theta > 0.0 && theta < ctrl->switch_theta) {
-    ctrl->switched = 1;
-    done = 1;
+  if (ctrl->switch_theta > 0.0) {  // mixed FP32 phase: the switch to FP64
+    const double tmin = ctrl->theta_min;
+    if (theta < tmin) ctrl->theta_min = theta;
+    const bool stalled = ctrl->stall_factor > 0.0 && tmin < ctrl->stall_theta && theta > ctrl->stall_factor * tmin;
+    if (theta < ctrl->switch_theta || stalled) {
+      ctrl->switched = stalled && !(theta < ctrl->switch_theta) ? 2 : 1;
+      done = 1;
+    }
   }
   ctrl->done = done;
   return done;

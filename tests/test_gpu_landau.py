@@ -205,6 +205,24 @@ def test_mixed_precision():
     assert rel(m["functional"], L.functional(W)) < 1e-10
 
 
+@pytest.mark.parametrize("stall", ["0", "30"])
+def test_mixed_stall_exit(stall, monkeypatch):
+    """The FP32 phase may hand over to FP64 at its first stall (the default
+    on lattices up to 2^16 sites) or run to the switch point: both end
+    converged, on FP64-preserved links."""
+    monkeypatch.setenv("GLU_GFIX_STALL_FACTOR", stall)
+    dims = (8, 8, 8, 8)
+    U, plaq = heatbath_config(dims, sweeps=20)
+    with fresh(dims, U, gl.MIXED) as g:
+        g.set_switch_theta(1e-11)
+        it, th = g.run(gl.CG, 0.08, 1e-14, 1000)
+        n32, n64 = g.mixed_split()
+        g.reunit()
+        m = g.measure()
+    assert th < 1e-14 and n32 > 0 and n64 > 0
+    assert abs(m["plaquette"] - plaq) < 1e-14
+
+
 def test_unit_and_pure_gauge():
     dims = (4, 4, 4, 8)
     V = int(np.prod(dims))
