@@ -163,6 +163,10 @@ int or_momentum_list(const or_lat *L, int type, size_t max_mom,
 int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
                   double cyl_width, double angle, int *mom, double *trans,
                   double *longi, size_t cap, size_t *n);
+/* exceptional-momentum two- and three-point functions g2(p), g3(p) */
+int or_gluon_exceptional(const or_lat *L, const double *U, int type,
+                         size_t max_mom, double cyl_width, double angle,
+                         int *mom, double *g2, double *g3, size_t cap, size_t *n);
 int or_write_glueprop(const char *path, size_t n, const int *mom,
                       const double *trans, const double *longi);
 

@@ -19,6 +19,9 @@
  *     exp(i p_mu / 2);
  *   - compute_gluon_prop and project_trans_long (Cuts/glueprop.c:32-151):
  *     the first half of the list and the zero, the rest by p -> -p;
+ *   - the exceptional two- and three-point functions of
+ *     write_exceptional_g2g3_MOMgg (Cuts/MOMgg.c:94-215), GLU's default
+ *     cuts output;
  *   - the output of write_mom_veclist and write_g2g3_to_list
  *     (Cuts/cut_output.c:30-52, 311-336, 371-400), big-endian.
  */
@@ -158,6 +161,48 @@ int or_momentum_list(const or_lat *L, int type, size_t max_mom,
   return in <= cap ? 0 : -1;
 }
 
+/* the Lie fields of cuts_struct up to the momentum-space Landau phase:
+   A[(mu * 9 + j) * V + x], transformed and corrected at the listed sites */
+static cplx *lie_fields(const or_lat *L, const double *U, size_t nm,
+                        const int *mom, const size_t *idx, double tw[OR_ND])
+{
+  const size_t V = L->V;
+  cplx *A = malloc(sizeof(cplx) * 36 * V);
+  for (size_t i = 0; i < V; i++)
+    for (int mu = 0; mu < OR_ND; mu++) {
+      cplx Q[9];
+      hermitian_proj(Q, (const cplx *)LINK(U, i, mu));
+      for (int j = 0; j < 9; j++) A[(size_t)(mu * 9 + j) * V + i] = Q[j];
+    }
+  or_fft((double *)A, L->dims, OR_ND, +1, 36);
+  for (int mu = 0; mu < OR_ND; mu++) tw[mu] = OR_TWOPI / (double)L->dims[mu];
+  /* correct_pspace_landau */
+  for (size_t i = 0; i < nm; i++)
+    for (int mu = 0; mu < OR_ND; mu++) {
+      const double cache = 0.5 * mom[i * OR_ND + mu] * tw[mu];
+      const double p_mu = 2.0 * sin(cache);
+      const cplx e = cos(cache) + I * 0.5 * p_mu;
+      for (int j = 0; j < 9; j++) A[(size_t)(mu * 9 + j) * V + idx[i]] *= e;
+    }
+  return A;
+}
+
+/* psq_calc from the listed momentum.  GLU recovers it from the site
+   through get_site_2piBZ (Geometry/geometry.c:42-61), the same map for
+   even extents; for an odd extent L it sends the site of k = (L-1)/2
+   through a negative coordinate and uses a neighbouring momentum. */
+static double psq_of(const int *k, const double tw[OR_ND], double p[OR_ND])
+{
+  double tspsq = 0.;
+  for (int mu = 0; mu < OR_ND; mu++) {
+    p[mu] = 2.0 * sin(k[mu] * tw[mu] * 0.5);
+    tspsq += p[mu] * p[mu];
+  }
+  return (tspsq < OR_PREC_TOL) ? 1.0 : tspsq;
+}
+
+#define AF(A, V, mu, j, site) (A)[(size_t)((mu) * 9 + (j)) * (V) + (site)]
+
 int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
                   double cyl_width, double angle, int *mom, double *trans,
                   double *longi, size_t cap, size_t *n)
@@ -170,50 +215,22 @@ int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
     return rc;
   }
   const size_t nm = *n;
-  /* A[f][x], f = mu * 9 + j */
-  cplx *A = malloc(sizeof(cplx) * 36 * V);
-  for (size_t i = 0; i < V; i++)
-    for (int mu = 0; mu < OR_ND; mu++) {
-      cplx Q[9];
-      hermitian_proj(Q, (const cplx *)LINK(U, i, mu));
-      for (int j = 0; j < 9; j++) A[(size_t)(mu * 9 + j) * V + i] = Q[j];
-    }
-  or_fft((double *)A, L->dims, OR_ND, +1, 36);
   double tw[OR_ND];
-  for (int mu = 0; mu < OR_ND; mu++) tw[mu] = OR_TWOPI / (double)L->dims[mu];
-  /* correct_pspace_landau */
-  for (size_t i = 0; i < nm; i++)
-    for (int mu = 0; mu < OR_ND; mu++) {
-      const double cache = 0.5 * mom[i * OR_ND + mu] * tw[mu];
-      const double p_mu = 2.0 * sin(cache);
-      const cplx e = cos(cache) + I * 0.5 * p_mu;
-      for (int j = 0; j < 9; j++) A[(size_t)(mu * 9 + j) * V + idx[i]] *= e;
-    }
+  cplx *A = lie_fields(L, U, nm, mom, idx, tw);
   const double g2_norm = 2.0 / ((OR_NC * OR_NC - 1) * (OR_ND - 1) * (double)V);
   const double g0_norm = 2.0 / ((OR_NC * OR_NC - 1) * (OR_ND) * (double)V);
   const double long_norm = 2.0 / ((OR_NC * OR_NC - 1) * (double)V);
   const size_t zero_pos = nm >> 1;
   for (size_t i = 0; i <= zero_pos && i < nm; i++) {
     const size_t posit = idx[i];
-    /* psq_calc: the momentum of the list entry.  GLU recovers it from the
-       site through get_site_2piBZ (Geometry/geometry.c:42-61), which is
-       the same map for even extents; for an odd extent L it sends the site
-       of k = (L-1)/2 through a negative coordinate and the projector of a
-       neighbouring momentum is used.  This uses the listed momentum. */
-    double p[OR_ND], tspsq = 0.;
-    for (int mu = 0; mu < OR_ND; mu++) {
-      const int k = mom[i * OR_ND + mu];
-      p[mu] = 2.0 * sin(k * tw[mu] * 0.5);
-      tspsq += p[mu] * p[mu];
-    }
-    const double spsq = 1.0 / ((tspsq < OR_PREC_TOL) ? 1.0 : tspsq);
+    double p[OR_ND];
+    const double spsq = 1.0 / psq_of(mom + i * OR_ND, tw, p);
     double tr_ = 0.0, lo = 0.0;
     for (int mu = 0; mu < OR_ND; mu++)
       for (int nu = 0; nu < OR_ND; nu++) {
         cplx tr = 0.0;
         for (int j = 0; j < 9; j++)
-          tr += A[(size_t)(mu * 9 + j) * V + posit] *
-                conj(A[(size_t)(nu * 9 + j) * V + posit]);
+          tr += AF(A, V, mu, j, posit) * conj(AF(A, V, nu, j, posit));
         const double common = p[mu] * p[nu] * spsq * creal(tr);
         tr_ += (mu != nu) ? -common : creal(tr) - common;
         lo += common;
@@ -224,6 +241,80 @@ int or_gluon_prop(const or_lat *L, const double *U, int type, size_t max_mom,
   for (size_t i = zero_pos + 1; i < nm; i++) {
     trans[i] = trans[nm - i - 1];
     longi[i] = longi[nm - i - 1];
+  }
+  free(A);
+  free(idx);
+  return 0;
+}
+
+/* 3x3 helpers of Matrix_Ops/U_Nops.c and trace_abc.c: c = a b, tr(a b) */
+static void mul33(cplx *c, const cplx *a, const cplx *b)
+{
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 3; s++)
+      c[r * 3 + s] = a[r * 3] * b[s] + a[r * 3 + 1] * b[3 + s] + a[r * 3 + 2] * b[6 + s];
+}
+static cplx tr_ab(const cplx *a, const cplx *b)
+{
+  cplx t = 0.0;
+  for (int r = 0; r < 3; r++)
+    for (int s = 0; s < 3; s++) t += a[r * 3 + s] * b[s * 3 + r];
+  return t;
+}
+
+/* write_exceptional_g2g3_MOMgg (Cuts/MOMgg.c:141-215) with the default
+   contraction_matrices (:94-121): for every listed p, with -p the mirror
+   entry of the list and 0 its middle,
+     g2 = sum_mu Re tr(A_mu(p) A_mu(-p)),
+     g3 = Re [tr(AA Arho) + tr(Arho AP AM)] / p^2 with AA = sum_mu A_mu(p)
+     A_mu(-p), Arho = sum_mu p_mu A_mu(0), AP = sum_mu p_mu A_mu(p),
+     AM = -sum_mu p_mu A_mu(-p) / p^2 */
+int or_gluon_exceptional(const or_lat *L, const double *U, int type,
+                         size_t max_mom, double cyl_width, double angle,
+                         int *mom, double *g2, double *g3, size_t cap, size_t *n)
+{
+  const size_t V = L->V;
+  size_t *idx = malloc(sizeof(size_t) * (cap ? cap : 1));
+  const int rc = or_momentum_list(L, type, max_mom, cyl_width, angle, mom, idx, cap, n);
+  if (rc) {
+    free(idx);
+    return rc;
+  }
+  const size_t nm = *n;
+  double tw[OR_ND];
+  cplx *A = lie_fields(L, U, nm, mom, idx, tw);
+  const double g2_norm = 2.0 / ((OR_NC * OR_NC - 1) * (OR_ND - 1) * (double)V);
+  const double g0_norm = 2.0 / ((OR_NC * OR_NC - 1) * (OR_ND) * (double)V);
+  const double g3_norm = 4.0 / (2.0 * OR_NC * (OR_NC * OR_NC - 1) * (OR_ND - 1) * (double)V);
+  const size_t zero = nm ? (nm - 1) / 2 : 0;
+  const size_t lzero = nm ? idx[zero] : 0;
+  for (size_t i = 0; i < nm; i++) {
+    const size_t posit = idx[i], cj = idx[nm - i - 1];
+    double s2 = 0.0;
+    cplx AA[9] = {0}, AM[9] = {0}, Arho[9] = {0}, AP[9] = {0};
+    double p[OR_ND];
+    const double spsq = psq_of(mom + i * OR_ND, tw, p);
+    for (int mu = 0; mu < OR_ND; mu++) {
+      cplx ap[9], am[9], a0[9], t[9];
+      for (int j = 0; j < 9; j++) {
+        ap[j] = AF(A, V, mu, j, posit);
+        am[j] = AF(A, V, mu, j, cj);
+        a0[j] = AF(A, V, mu, j, lzero);
+      }
+      s2 += creal(tr_ab(ap, am));
+      mul33(t, ap, am);
+      for (int j = 0; j < 9; j++) {
+        AA[j] += t[j];
+        Arho[j] += a0[j] * p[mu];
+        AP[j] += ap[j] * p[mu];
+        AM[j] += am[j] * (-p[mu] / spsq);
+      }
+    }
+    g2[i] = s2 * (i != zero ? g2_norm : g0_norm);
+    cplx t[9];
+    mul33(t, Arho, AP);
+    const cplx trans = tr_ab(AA, Arho), wiv = tr_ab(t, AM);
+    g3[i] = creal((trans + wiv) / spsq) * g3_norm;
   }
   free(A);
   free(idx);

@@ -101,6 +101,8 @@ def lib():
             "or_gluon_prop": (C.c_int, [LP, DP, C.c_int, SZ, C.c_double, C.c_double, IP, DP, DP, SZ,
                                         C.POINTER(SZ)]),
             "or_write_glueprop": (C.c_int, [C.c_char_p, SZ, IP, DP, DP]),
+            "or_gluon_exceptional": (C.c_int, [LP, DP, C.c_int, SZ, C.c_double, C.c_double, IP, DP, DP, SZ,
+                                               C.POINTER(SZ)]),
             "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.c_int]),
         }
@@ -294,6 +296,21 @@ def gluon_prop(lat: Lattice, U: np.ndarray, cut="psq", max_mom=4, cyl_width=1.0,
     assert rc == 0
     k = n.value
     return mom[:k], G[:k], F[:k]
+
+
+def gluon_exceptional(lat: Lattice, U: np.ndarray, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0):
+    """GLU's exceptional two- and three-point functions: (mom[n, 4], g2[n], g3[n])."""
+    mom0, _ = momentum_list(lat, cut, max_mom, cyl_width, angle)
+    n = SZ()
+    cap = max(len(mom0), 1)
+    mom = np.zeros((cap, 4), np.int32)
+    g2, g3 = np.zeros(cap), np.zeros(cap)
+    rc = lib().or_gluon_exceptional(lat.ref, _dp(np.ascontiguousarray(U)),
+                                    CUTS[cut] if isinstance(cut, str) else cut, max_mom, cyl_width, angle,
+                                    mom.ctypes.data_as(IP), _dp(g2), _dp(g3), cap, C.byref(n))
+    assert rc == 0
+    k = n.value
+    return mom[:k], g2[:k], g3[:k]
 
 
 def write_glueprop(path: str, mom: np.ndarray, G: np.ndarray, F: np.ndarray):

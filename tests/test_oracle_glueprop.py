@@ -178,3 +178,72 @@ def test_write_glueprop_format(tmp_path):
     assert np.frombuffer(b[o:o + 4], ">u4")[0] == 3
     assert np.array_equal(np.frombuffer(b[o + 4:o + 28], ">f8"), F)
     assert len(b) == o + 28
+
+
+# ------------------------------------------ exceptional two/three point ---
+def test_exceptional_g2_is_the_propagator_trace():
+    """With A(-p) = A(p)^dag, sum_mu tr A_mu(p) A_mu(-p) is the trace the
+    projectors split: g2 = G + F / (ND - 1) at p != 0 and g2 = G at p = 0."""
+    dims = (4, 4, 4, 6)
+    rng = np.random.default_rng(7)
+    V = int(np.prod(dims))
+    U = to_glu(haar(rng, V * 4).reshape(V, 4, 3, 3))
+    L = O.Lattice(dims, nth=2)
+    m1, G, F = O.gluon_prop(L, U, "psq", 3)
+    m2, g2, g3 = O.gluon_exceptional(L, U, "psq", 3)
+    assert np.array_equal(m1, m2)
+    z = len(g2) // 2
+    want = G + F / 3.0
+    want[z] = G[z]
+    assert np.abs(g2 - want).max() < 1e-12 * np.abs(want).max()
+    assert g3[z] == 0.0  # p = 0: the rho sum vanishes
+
+
+def test_exceptional_is_colour_rotation_invariant():
+    """U -> W U W^dag with one constant W in SU(3) rotates every A_mu(p) the
+    same way; g2 and g3 are traces of products and do not change."""
+    dims = (4, 4, 4, 6)
+    rng = np.random.default_rng(8)
+    V = int(np.prod(dims))
+    M = haar(rng, V * 4).reshape(V, 4, 3, 3)
+    W = haar(rng, 1)[0]
+    R = W @ M @ W.conj().T
+    L = O.Lattice(dims, nth=2)
+    _, a2, a3 = O.gluon_exceptional(L, to_glu(M), "psq", 3)
+    _, b2, b3 = O.gluon_exceptional(L, to_glu(R), "psq", 3)
+    assert np.abs(a2 - b2).max() < 1e-12 * np.abs(a2).max()
+    assert np.abs(a3 - b3).max() < 1e-12 * np.abs(a3).max()
+    assert np.abs(a3).max() > 1e-6  # not trivially zero
+
+
+def test_exceptional_g3_against_numpy():
+    """The field pipeline (lists, transform, Landau phase, sites of -p and
+    0) by numpy; the contraction is MOMgg.c's contraction_matrices written
+    out again, so this pins the pipeline around it, not the formula."""
+    dims = (4, 4, 4, 6)
+    rng = np.random.default_rng(9)
+    V = int(np.prod(dims))
+    U = to_glu(haar(rng, V * 4).reshape(V, 4, 3, 3))
+    mom, g2, g3 = O.gluon_exceptional(O.Lattice(dims, nth=2), U, "psq", 3)
+    M = (U[..., 0::2] + 1j * U[..., 1::2]).reshape(V, 4, 3, 3)
+    A = (M - np.conj(np.swapaxes(M, -1, -2))) / 2j
+    A = A - np.trace(A, axis1=-2, axis2=-1)[..., None, None] * np.eye(3) / 3
+    Ft = (np.fft.ifftn(A.reshape(dims[::-1] + (4, 3, 3)), axes=(0, 1, 2, 3)) * V).reshape(V, 4, 3, 3)
+    Lv = np.array(dims)
+
+    def at(k):
+        site = int(sum((k[mu] % dims[mu]) * np.prod(dims[:mu]) for mu in range(4)))
+        return Ft[site] * np.exp(1j * np.pi * k / Lv)[:, None, None]
+
+    A0 = at(np.zeros(4, int))
+    for i, k in enumerate(mom):
+        Ap, Am = at(k), at(-k)
+        p = 2 * np.sin(np.pi * k / Lv)
+        p2 = (p ** 2).sum()
+        p2 = p2 if p2 >= 3e-14 else 1.0
+        AA = sum(Ap[m] @ Am[m] for m in range(4))
+        Arho = sum(p[m] * A0[m] for m in range(4))
+        AP = sum(p[m] * Ap[m] for m in range(4))
+        AM = sum(-p[m] / p2 * Am[m] for m in range(4))
+        want = ((np.trace(AA @ Arho) + np.trace(Arho @ AP @ AM)) / p2).real * 4 / (2 * 3 * 8 * 3 * V)
+        assert abs(g3[i] - want) < 1e-12 * max(np.abs(g3).max(), 1e-300)
