@@ -35,7 +35,7 @@ EXPORTS = [
     "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
-    "glu_gfix_gluon_props",
+    "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
@@ -71,6 +71,8 @@ def lib():
             "glu_gfix_write_nersc": (C.c_int, [P, C.c_char_p, C.c_int, C.c_int, C.c_char_p, SZ]),
             "glu_gfix_gluon_props": (C.c_int, [P, C.c_int, SZ, C.c_double, C.c_double, C.c_char_p,
                                                C.POINTER(SZ), C.POINTER(C.c_int), DP, DP, SZ, DP]),
+            "glu_gfix_gluon_exceptional": (C.c_int, [P, C.c_int, SZ, C.c_double, C.c_double, C.c_char_p,
+                                                     C.POINTER(SZ), C.POINTER(C.c_int), DP, DP, SZ, DP]),
             "glu_gfix_mixed_split": (C.c_int, [P, C.POINTER(SZ), C.POINTER(SZ)]),
             "glu_gfix_slice_info": (C.c_int, [P, C.POINTER(SZ), DP]),
             "glu_gfix_stage_deriv": (C.c_int, [P, DP, DP]),
@@ -217,21 +219,28 @@ class Gfix:
     def reunit(self):
         _check(lib().glu_gfix_reunit(self._h), "reunit")
 
+    def _gluon_cuts(self, fn, cut, max_mom, cyl_width, angle, path):
+        t = CUTS[cut] if isinstance(cut, str) else int(cut)
+        n = SZ()
+        _check(fn(self._h, t, max_mom, cyl_width, angle, None, C.byref(n), None, None, None, 0, None),
+               fn.__name__)
+        k = n.value
+        mom = np.zeros((k, 4), np.int32)
+        a, b, err = np.zeros(k), np.zeros(k), C.c_double()
+        _check(fn(self._h, t, max_mom, cyl_width, angle, None if path is None else os.fspath(path).encode(),
+                  C.byref(n), mom.ctypes.data_as(C.POINTER(C.c_int)), _dp(a), _dp(b), k, C.byref(err)),
+               fn.__name__)
+        return mom, a, b, err.value
+
     def gluon_props(self, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0, path=None):
         """GLU's GLUON_PROPS cuts on the device links: (mom[n, 4], G[n], F[n],
         momentum-space Landau-condition figure)."""
-        t = CUTS[cut] if isinstance(cut, str) else int(cut)
-        n = SZ()
-        _check(lib().glu_gfix_gluon_props(self._h, t, max_mom, cyl_width, angle, None, C.byref(n), None, None,
-                                          None, 0, None), "gluon_props")
-        k = n.value
-        mom = np.zeros((k, 4), np.int32)
-        G, F, err = np.zeros(k), np.zeros(k), C.c_double()
-        _check(lib().glu_gfix_gluon_props(self._h, t, max_mom, cyl_width, angle,
-                                          None if path is None else os.fspath(path).encode(), C.byref(n),
-                                          mom.ctypes.data_as(C.POINTER(C.c_int)), _dp(G), _dp(F), k,
-                                          C.byref(err)), "gluon_props")
-        return mom, G, F, err.value
+        return self._gluon_cuts(lib().glu_gfix_gluon_props, cut, max_mom, cyl_width, angle, path)
+
+    def gluon_exceptional(self, cut="psq", max_mom=4, cyl_width=1.0, angle=20.0, path=None):
+        """GLU's default (EXCEPTIONAL) cuts output on the device links:
+        (mom[n, 4], g2[n], g3[n], momentum-space Landau-condition figure)."""
+        return self._gluon_cuts(lib().glu_gfix_gluon_exceptional, cut, max_mom, cyl_width, angle, path)
 
     def write_nersc(self, path: str, two_rows: bool = False, big_endian: bool = True, label: str = "",
                     sequence: int = 1):

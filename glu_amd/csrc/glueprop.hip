@@ -58,6 +58,42 @@ __device__ __forceinline__ double2 cmulz(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 conjz(double2 a) { return make_double2(a.x, -a.y); }
 
+// the listed momentum tables: axis mu of entry j -> (2 sin(k pi/L), phase)
+struct MomTabs {
+  const int *mom;
+  const long long *site, *msite;
+  const double *sin2, *cosk;
+  int off[4], half[4];
+};
+
+// the nine entries of A_mu at listed entry j from the packed transforms at
+// its site and the site of -p, times the Landau phase exp(i p_mu / 2) of
+// correct_pspace_landau (Cuts/pspace_landau.c:28-86); p_mu -> *pmu
+__device__ __forceinline__ void lie_at(const double2 *__restrict__ Z, long long V, const MomTabs &m, int j, int mu,
+                                       double2 (&A)[9], double *pmu) {
+  const int t = m.off[mu] + m.mom[4 * j + mu] + m.half[mu];
+  *pmu = m.sin2[t];
+  const double2 e = make_double2(m.cosk[t], 0.5 * m.sin2[t]);
+  const long long s = m.site[j], ms = m.msite[j];
+  const double2 *z = Z + (long long)mu * 4 * V;
+  const double2 z0p = z[s], z0m = z[ms];
+  // diagonal: a00 = (Z0(p) + conj Z0(-p)) / 2, a11 = (Z0(p) - conj Z0(-p)) / 2i
+  const double2 a00 = make_double2(0.5 * (z0p.x + z0m.x), 0.5 * (z0p.y - z0m.y));
+  const double2 a11 = make_double2(0.5 * (z0p.y + z0m.y), -0.5 * (z0p.x - z0m.x));
+  A[0] = a00;
+  A[4] = a11;
+  A[8] = make_double2(-a00.x - a11.x, -a00.y - a11.y);
+  // upper entries at p, lower ones from conj of the upper at -p
+  A[1] = z[V + s];
+  A[2] = z[2 * V + s];
+  A[5] = z[3 * V + s];
+  A[3] = conjz(z[V + ms]);
+  A[6] = conjz(z[2 * V + ms]);
+  A[7] = conjz(z[3 * V + ms]);
+#pragma unroll
+  for (int k = 0; k < 9; k++) A[k] = cmulz(A[k], e);
+}
+
 // One listed momentum per thread: the nine entries of every A_mu(p) from
 // the packed transforms at p and -p, the exp(i p_mu / 2) factor of
 // correct_pspace_landau (Cuts/pspace_landau.c:28-86), and
@@ -68,45 +104,20 @@ __device__ __forceinline__ double2 conjz(double2 a) { return make_double2(a.x, -
 // for even extents; for an odd extent it misplaces k = (L-1)/2).  err
 // receives the momentum-space gauge condition term
 // |sum_mu,j p_mu A_mu,j(p)| / NC.
-__global__ __launch_bounds__(kBlock) void k_gluon_prop(const double2 *__restrict__ Z, long long V, int n,
-                                                      const int *__restrict__ mom,
-                                                      const long long *__restrict__ site,
-                                                      const long long *__restrict__ msite,
-                                                      const double *__restrict__ sin2,
-                                                      const double *__restrict__ cosk, int4 off, int4 half,
+__global__ __launch_bounds__(kBlock) void k_gluon_prop(const double2 *__restrict__ Z, long long V, int n, MomTabs m,
                                                       int zero_pos, double g2_norm, double g0_norm, double long_norm,
                                                       double *__restrict__ G, double *__restrict__ F,
                                                       double *__restrict__ err) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const long long s = site[j], ms = msite[j];
-  const int offs[4] = {off.x, off.y, off.z, off.w}, hl[4] = {half.x, half.y, half.z, half.w};
   double p[4];
   double2 A[4][9];
   double2 esum = make_double2(0.0, 0.0);
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
-    const int t = offs[mu] + mom[4 * j + mu] + hl[mu];
-    p[mu] = sin2[t];
-    const double2 e = make_double2(cosk[t], 0.5 * sin2[t]);
-    const double2 *z = Z + (long long)mu * 4 * V;
-    const double2 z0p = z[s], z0m = z[ms];
-    // diagonal: a00 = (Z0(p) + conj Z0(-p)) / 2, a11 = (Z0(p) - conj Z0(-p)) / 2i
-    const double2 a00 = make_double2(0.5 * (z0p.x + z0m.x), 0.5 * (z0p.y - z0m.y));
-    const double2 a11 = make_double2(0.5 * (z0p.y + z0m.y), -0.5 * (z0p.x - z0m.x));
-    A[mu][0] = a00;
-    A[mu][4] = a11;
-    A[mu][8] = make_double2(-a00.x - a11.x, -a00.y - a11.y);
-    // upper entries at p, lower ones from conj of the upper at -p
-    A[mu][1] = z[V + s];
-    A[mu][2] = z[2 * V + s];
-    A[mu][5] = z[3 * V + s];
-    A[mu][3] = conjz(z[V + ms]);
-    A[mu][6] = conjz(z[2 * V + ms]);
-    A[mu][7] = conjz(z[3 * V + ms]);
+    lie_at(Z, V, m, j, mu, A[mu], &p[mu]);
 #pragma unroll
     for (int k = 0; k < 9; k++) {
-      A[mu][k] = cmulz(A[mu][k], e);
       esum.x += A[mu][k].x * p[mu];
       esum.y += A[mu][k].y * p[mu];
     }
@@ -129,6 +140,98 @@ __global__ __launch_bounds__(kBlock) void k_gluon_prop(const double2 *__restrict
     }
   G[j] = tr_ * (j == zero_pos ? g0_norm : g2_norm);
   F[j] = lo * long_norm;
+  err[j] = sqrt(esum.x * esum.x + esum.y * esum.y) / 3.0;
+}
+
+__device__ __forceinline__ void mul33(double2 (&c)[9], const double2 (&a)[9], const double2 (&b)[9]) {
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      double2 t = cmulz(a[r * 3], b[s]);
+      const double2 u = cmulz(a[r * 3 + 1], b[3 + s]), v = cmulz(a[r * 3 + 2], b[6 + s]);
+      t.x += u.x + v.x;
+      t.y += u.y + v.y;
+      c[r * 3 + s] = t;
+    }
+}
+// tr(a b), trace_ab (Matrix_Ops/U_Nops.c:799-807)
+__device__ __forceinline__ double2 tr_ab(const double2 (&a)[9], const double2 (&b)[9]) {
+  double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const double2 u = cmulz(a[r * 3 + s], b[s * 3 + r]);
+      t.x += u.x;
+      t.y += u.y;
+    }
+  return t;
+}
+
+// write_exceptional_g2g3_MOMgg (Cuts/MOMgg.c:141-215) with its default
+// contraction_matrices (:94-121), one listed momentum p per thread; -p is
+// the mirror entry of the list and 0 its middle entry:
+//   g2 = sum_mu Re tr(A_mu(p) A_mu(-p)),
+//   g3 = Re [tr(AA Arho) + tr(Arho AP AM)] / p^2, AA = sum_mu A_mu(p)
+//   A_mu(-p), Arho = sum_mu p_mu A_mu(0), AP = sum_mu p_mu A_mu(p),
+//   AM = -sum_mu p_mu A_mu(-p) / p^2.
+__global__ __launch_bounds__(kBlock) void k_gluon_exceptional(const double2 *__restrict__ Z, long long V, int n,
+                                                             MomTabs m, double g2_norm, double g0_norm,
+                                                             double g3_norm, double *__restrict__ g2,
+                                                             double *__restrict__ g3, double *__restrict__ err) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int cj = n - 1 - j, zero = (n - 1) / 2;
+  double p[4], spsq = 0.0;
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    p[mu] = m.sin2[m.off[mu] + m.mom[4 * j + mu] + m.half[mu]];
+    spsq += p[mu] * p[mu];
+  }
+  if (spsq < kPrecTol) spsq = 1.0;
+  double2 Arho[9], AA[9], AP[9], AM[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Arho[k] = AA[k] = AP[k] = AM[k] = make_double2(0.0, 0.0);
+#pragma unroll 1
+  for (int mu = 0; mu < 4; mu++) {
+    double2 a0[9];
+    double q;
+    lie_at(Z, V, m, zero, mu, a0, &q);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      Arho[k].x += a0[k].x * p[mu];
+      Arho[k].y += a0[k].y * p[mu];
+    }
+  }
+  double s2 = 0.0;
+  double2 esum = make_double2(0.0, 0.0);
+#pragma unroll 1
+  for (int mu = 0; mu < 4; mu++) {
+    double2 ap[9], am[9], t[9];
+    double q;
+    lie_at(Z, V, m, j, mu, ap, &q);
+    lie_at(Z, V, m, cj, mu, am, &q);
+    s2 += tr_ab(ap, am).x;
+    mul33(t, ap, am);
+    const double wm = -p[mu] / spsq;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      AA[k].x += t[k].x;
+      AA[k].y += t[k].y;
+      AP[k].x += ap[k].x * p[mu];
+      AP[k].y += ap[k].y * p[mu];
+      AM[k].x += am[k].x * wm;
+      AM[k].y += am[k].y * wm;
+      esum.x += ap[k].x * p[mu];
+      esum.y += ap[k].y * p[mu];
+    }
+  }
+  g2[j] = s2 * (j != zero ? g2_norm : g0_norm);
+  double2 t[9];
+  mul33(t, Arho, AP);
+  const double2 trans = tr_ab(AA, Arho), wiv = tr_ab(t, AM);
+  g3[j] = (trans.x + wiv.x) / spsq * g3_norm;
   err[j] = sqrt(esum.x * esum.x + esum.y * esum.y) / 3.0;
 }
 
@@ -370,11 +473,14 @@ int glue_device(glu_gfix_ctx *c, const ListKey &key, const List &list) {
 
 }  // namespace
 
-extern "C" int glu_gfix_gluon_props(glu_gfix_ctx *c, int cut, size_t max_mom, double cyl_width, double angle,
-                                    const char *path, size_t *nmom, int *mom, double *transverse,
-                                    double *longitudinal, size_t cap, double *pspace_err) {
-  if (!c || !nmom) return gfix_fail(-1, "glu_gfix_gluon_props: null argument");
-  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "gluon propagators need a Landau context (4D transforms)");
+// the GLUON_PROPS (exceptional = 0) or EXCEPTIONAL (1) branch of
+// cuts_struct (Cuts/cuts.c:114-184) on the device links: out1 / out2 are
+// G / F or g2 / g3
+static int gluon_cuts(glu_gfix_ctx *c, int exceptional, int cut, size_t max_mom, double cyl_width, double angle,
+                      const char *path, size_t *nmom, int *mom, double *out1, double *out2, size_t cap,
+                      double *pspace_err) {
+  if (!c || !nmom) return gfix_fail(-1, "gluon cuts: null argument");
+  if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "gluon cuts need a Landau context (4D transforms)");
   if (c->run.inflight) return gfix_fail(-1, "a run is in flight on this context");
   List list;
   if (cached_list(c->dims, cut, max_mom, cyl_width, angle, list))
@@ -383,12 +489,13 @@ extern "C" int glu_gfix_gluon_props(glu_gfix_ctx *c, int cut, size_t max_mom, do
   const std::vector<long long> &site = list.site;
   const int n = (int)site.size();
   *nmom = (size_t)n;
-  if ((mom || transverse || longitudinal) && cap < (size_t)n)
+  if ((mom || out1 || out2) && cap < (size_t)n)
     return gfix_fail(-1, "momentum list has %d entries, the output arrays %zu", n, cap);
   if (n == 0) return gfix_fail(-1, "empty momentum list");
-  if (!mom && !transverse && !longitudinal && !path && !pspace_err) return 0;  // the count only
+  if (!mom && !out1 && !out2 && !path && !pspace_err) return 0;  // the count only
   // GLU checks that list[i] = -list[n - 1 - i] and warns otherwise
-  // (Cuts/cut_routines.c:415-437); its p -> -p fill is then not exact
+  // (Cuts/cut_routines.c:415-437); its use of the mirror entry as -p is
+  // then not exact
   for (int i = 0; i < n; i++) {
     bool sym = true;
     for (int mu = 0; mu < 4; mu++) sym = sym && km[4 * i + mu] + km[4 * (n - 1 - i) + mu] == 0;
@@ -402,12 +509,18 @@ extern "C" int glu_gfix_gluon_props(glu_gfix_ctx *c, int cut, size_t max_mom, do
   // the 16 packed fields stay with the context for later measurements
   if (!c->lie && hipMalloc(&c->lie, (size_t)V * 16 * sizeof(double2)) != hipSuccess) {
     c->lie = nullptr;
-    return gfix_fail(-2, "gluon propagators: field allocation failed");
+    return gfix_fail(-2, "gluon cuts: field allocation failed");
   }
   const GlueDev &d = *(const GlueDev *)c->glue;
+  MomTabs tabs{d.mom, d.site, d.msite, d.tab, d.tab + d.ntab, {}, {}};
+  for (int mu = 0; mu < 4; mu++) {
+    tabs.off[mu] = d.off[mu];
+    tabs.half[mu] = d.half[mu];
+  }
   const double g2_norm = 2.0 / ((3 * 3 - 1) * (4 - 1) * (double)V);
   const double g0_norm = 2.0 / ((3 * 3 - 1) * (4) * (double)V);
   const double long_norm = 2.0 / ((3 * 3 - 1) * (double)V);
+  const double g3_norm = 4.0 / (2.0 * 3 * (3 * 3 - 1) * (4 - 1) * (double)V);
   hipStream_t s = c->stream;
   void *Z = c->lie;
   GFIX_PROF(c, PK_LIE, hipLaunchKernelGGL(k_lie_pack, dim3((unsigned)c->nblk), dim3(kBlock), 0, s,
@@ -421,34 +534,52 @@ extern "C" int glu_gfix_gluon_props(glu_gfix_ctx *c, int cut, size_t max_mom, do
   for (int a = xy ? 2 : 0; a < 4; a++)
     GFIX_PROF(c, PK_FFT, launch_fft_axis<double>(Z, V, 16, c->ax[a], 1, c->ps, nullptr, s));
   double *out = d.out;
-  GFIX_PROF(c, PK_PROP,
-            hipLaunchKernelGGL(k_gluon_prop, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
-                               (const double2 *)Z, V, n, d.mom, d.site, d.msite, d.tab, d.tab + d.ntab,
-                               make_int4(d.off[0], d.off[1], d.off[2], d.off[3]),
-                               make_int4(d.half[0], d.half[1], d.half[2], d.half[3]), n >> 1, g2_norm, g0_norm,
-                               long_norm, out, out + n, out + 2 * n));
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  if (exceptional)
+    GFIX_PROF(c, PK_PROP,
+              hipLaunchKernelGGL(k_gluon_exceptional, grid, dim3(kBlock), 0, s, (const double2 *)Z, V, n, tabs,
+                                 g2_norm, g0_norm, g3_norm, out, out + n, out + 2 * n));
+  else
+    GFIX_PROF(c, PK_PROP,
+              hipLaunchKernelGGL(k_gluon_prop, grid, dim3(kBlock), 0, s, (const double2 *)Z, V, n, tabs, n >> 1,
+                                 g2_norm, g0_norm, long_norm, out, out + n, out + 2 * n));
   std::vector<double> h((size_t)n * 3);
   if (hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
-    return gfix_fail(-2, "gluon propagators failed: %s", hipGetErrorString(hipGetLastError()));
+    return gfix_fail(-2, "gluon cuts failed: %s", hipGetErrorString(hipGetLastError()));
   if (c->prof) prof_harvest(c);
-  std::vector<double> G(h.begin(), h.begin() + n), F(h.begin() + n, h.begin() + 2 * n);
-  // compute_gluon_prop fills the second half from the first by p -> -p
-  // (Cuts/glueprop.c:139-143)
-  const int zero_pos = n >> 1;
-  for (int i = zero_pos + 1; i < n; i++) {
-    G[i] = G[n - i - 1];
-    F[i] = F[n - i - 1];
+  std::vector<double> A(h.begin(), h.begin() + n), B(h.begin() + n, h.begin() + 2 * n);
+  if (!exceptional) {
+    // compute_gluon_prop fills the second half from the first by p -> -p
+    // (Cuts/glueprop.c:139-143)
+    const int zero_pos = n >> 1;
+    for (int i = zero_pos + 1; i < n; i++) {
+      A[i] = A[n - i - 1];
+      B[i] = B[n - i - 1];
+    }
   }
   double e = 0.0;
   for (int i = 0; i < n; i++) e += h[2 * (size_t)n + i];
   if (pspace_err) *pspace_err = e / (double)n;
   if (c->verbose) fprintf(stdout, "\n[CUTS] P-Space gauge fixing error :: %1.5e \n\n", e / (double)n);
   if (mom) memcpy(mom, km.data(), (size_t)n * 4 * sizeof(int));
-  if (transverse) memcpy(transverse, G.data(), (size_t)n * sizeof(double));
-  if (longitudinal) memcpy(longitudinal, F.data(), (size_t)n * sizeof(double));
-  if (path && write_props(path, km, G, F)) return gfix_fail(-3, "cannot write %s", path);
+  if (out1) memcpy(out1, A.data(), (size_t)n * sizeof(double));
+  if (out2) memcpy(out2, B.data(), (size_t)n * sizeof(double));
+  if (path && write_props(path, km, A, B)) return gfix_fail(-3, "cannot write %s", path);
   return 0;
+}
+
+extern "C" int glu_gfix_gluon_props(glu_gfix_ctx *c, int cut, size_t max_mom, double cyl_width, double angle,
+                                    const char *path, size_t *nmom, int *mom, double *transverse,
+                                    double *longitudinal, size_t cap, double *pspace_err) {
+  return gluon_cuts(c, 0, cut, max_mom, cyl_width, angle, path, nmom, mom, transverse, longitudinal, cap,
+                    pspace_err);
+}
+
+extern "C" int glu_gfix_gluon_exceptional(glu_gfix_ctx *c, int cut, size_t max_mom, double cyl_width, double angle,
+                                          const char *path, size_t *nmom, int *mom, double *g2, double *g3,
+                                          size_t cap, double *pspace_err) {
+  return gluon_cuts(c, 1, cut, max_mom, cyl_width, angle, path, nmom, mom, g2, g3, cap, pspace_err);
 }
 
 void glueprop_free(glu_gfix_ctx *c) {

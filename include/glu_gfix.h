@@ -189,6 +189,15 @@ int glu_gfix_gluon_props(glu_gfix_ctx *ctx, int cut, size_t max_mom, double cyl_
                          const char *path, size_t *nmom, int *mom, double *transverse,
                          double *longitudinal, size_t cap, double *pspace_err);
 
+/* GLU's default cuts output, the EXCEPTIONAL branch of cuts_struct: the
+   gluon two- and three-point functions g2(p), g3(p) at exceptional
+   momenta (p, -p, 0) of write_exceptional_g2g3_MOMgg (Cuts/MOMgg.c:141-215)
+   with its default contraction_matrices (:94-121).  Arguments, list and
+   file as for glu_gfix_gluon_props, with g2 / g3 in place of G / F. */
+int glu_gfix_gluon_exceptional(glu_gfix_ctx *ctx, int cut, size_t max_mom, double cyl_width, double angle,
+                               const char *path, size_t *nmom, int *mom, double *g2, double *g3,
+                               size_t cap, double *pspace_err);
+
 /* Write the device links as a NERSC file: the NERSC branch of write_lat
    (IO/writers.c:330-422) with write_header_NERSC (IO/write_headers.c:77-138).
    As in GLU the links are reunitarised in place first.  two_rows selects

@@ -32,7 +32,9 @@
  * matrix products; the line search by the functional after trial updates;
  * the loops by pure-gauge inputs and invariants; the gluon propagators
  * (tests/test_oracle_glueprop.py) by brute-force momentum lists, numpy and
- * a plane wave.  Parity unpinned: the
+ * a plane wave; the exceptional g2 by the propagators' trace, g3 by
+ * colour-rotation invariance and numpy around GLU's contraction formula
+ * (partly pinned).  Parity unpinned: the
  * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
  * trajectories, and or_heatbath, which is an independent generator (a
  * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.

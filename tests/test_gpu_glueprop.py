@@ -112,3 +112,37 @@ def test_gluon_props_refusals():
     with gl.Gfix((4, 4, 4, 8), gl.COULOMB) as g:
         with pytest.raises(gl.GfixError, match="Landau context"):
             g.gluon_props("psq", 3)
+
+
+@pytest.mark.parametrize("cut,max_mom", [("psq", 9), ("cylinder", 12)])
+def test_gluon_exceptional_matches_the_oracle(fixed, cut, max_mom):
+    """GLU's default cuts output (MOMgg exceptional g2, g3) on the Landau-
+    fixed field: g3 to 1e-12 of its largest value, g2 likewise and equal to
+    the propagators' G + F / 3 (their common trace)."""
+    dims, W, _ = fixed
+    print(f"gluon_exceptional {dims} cut={cut} max_mom={max_mom}", flush=True)
+    mo, g2o, g3o = O.gluon_exceptional(O.Lattice(dims, nth=4), W, cut, max_mom, 1.0, 20.0)
+    with gl.Gfix(dims) as g:
+        g.set_links(W)
+        mg, g2, g3, _ = g.gluon_exceptional(cut, max_mom, 1.0, 20.0)
+        _, G, F, _ = g.gluon_props(cut, max_mom, 1.0, 20.0)
+    assert np.array_equal(mg, mo)
+    assert np.abs(g2 - g2o).max() < 1e-12 * np.abs(g2o).max()
+    assert np.abs(g3 - g3o).max() < 1e-12 * np.abs(g3o).max()
+    assert np.abs(g3o).max() > 0
+    z = len(g2) // 2
+    want = G + F / 3.0
+    want[z] = G[z]
+    assert np.abs(g2 - want).max() < 1e-12 * np.abs(want).max()
+
+
+def test_gluon_exceptional_of_an_unfixed_field():
+    dims = (6, 6, 8, 8)
+    U = haar_links(dims, seed=6)
+    mo, g2o, g3o = O.gluon_exceptional(O.Lattice(dims, nth=2), U, "psq", 3)
+    with gl.Gfix(dims) as g:
+        g.set_links(U)
+        mg, g2, g3, _ = g.gluon_exceptional("psq", 3)
+    assert np.array_equal(mg, mo)
+    assert np.abs(g2 - g2o).max() < 1e-12 * np.abs(g2o).max()
+    assert np.abs(g3 - g3o).max() < 1e-12 * np.abs(g3o).max()
