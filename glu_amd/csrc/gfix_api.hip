@@ -33,7 +33,7 @@ namespace {
 constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-455
 // mixed precision: lattices up to this many sites stop the FP32 phase at
 // its first stall (theta 30x above its minimum once below 1e-6)
-constexpr long long kStallSites = 1 << 16;
+constexpr long long kStallSites = 1 << 17;
 constexpr long long kBatch = 8;        // iterations enqueued between host checks
 constexpr int kGluFailures = 7;        // GF_GLU_FAILURES, Headers/GLU_definitions.h:400-402
 

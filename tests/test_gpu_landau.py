@@ -208,7 +208,7 @@ def test_mixed_precision():
 @pytest.mark.parametrize("stall", ["0", "30"])
 def test_mixed_stall_exit(stall, monkeypatch):
     """The FP32 phase may hand over to FP64 at its first stall (the default
-    on lattices up to 2^16 sites) or run to the switch point: both end
+    on lattices up to 2^17 sites) or run to the switch point: both end
     converged, on FP64-preserved links."""
     monkeypatch.setenv("GLU_GFIX_STALL_FACTOR", stall)
     dims = (8, 8, 8, 8)
