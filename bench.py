@@ -478,12 +478,15 @@ def serve_main(a):
         lats = [r[2] for r in res]
         # a restarted request reports 123456789 + ..., as Landau() does
         restarted = sum(1 for r in res if r[0] >= 123456789)
+        which = sorted("x".join(map(str, picks[k][0])) + f"@{picks[k][1]}" for k, r in enumerate(res)
+                       if r[0] >= 123456789)
         its = [r[0] for r in res if r[0] < 123456789]
         if not all(r[1] < 1e-14 for r in res):
             raise RuntimeError("a request did not converge")
         return {"configs_per_s": round(n / wall, 3), "wall_s": round(wall, 2),
                 "p50_ms": round(pct(lats, 0.5), 1), "p99_ms": round(pct(lats, 0.99), 1),
-                "mean_iters": round(sum(its) / max(len(its), 1), 1), "restarted": restarted}
+                "mean_iters": round(sum(its) / max(len(its), 1), 1), "restarted": restarted,
+                "restarted_requests": which}
 
     runs = {}
     for slots in a.slots:
