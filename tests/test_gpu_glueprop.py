@@ -61,7 +61,7 @@ def test_gluon_props_match_the_oracle(fixed, cut, max_mom):
     assert np.array_equal(mg[::-1], -mg)
     assert np.abs(Gg - Go).max() < 1e-12 * np.abs(Go).max()
     # in Landau gauge p_mu A_mu(p) = 0, so F is rounding noise on the scale
-    # of G: both sides are judged on that scale
+    # of G: both sides are compared on that scale
     assert np.abs(Fg - Fo).max() < 1e-12 * np.abs(Go).max()
     assert np.abs(Fg).max() < 1e-12 * np.abs(Gg).max()
     # GLU's momentum-space Landau-condition figure; the unfixed field is far off
