@@ -368,6 +368,10 @@ def glueprop_main(a):
             kern[name]["GBps"] = round(GLUEPROP_ALG[name] * V / (mean * 1e-3) / 1e9, 1)
     dom = max(GLUEPROP_ALG, key=lambda k: kern.get(k, {}).get("total_ms", 0.0))
     ach = GLUEPROP_ALG[dom] * V / (kern[dom]["mean_ms"] * 1e-3) / 1e9
+    # HBM bytes per launch from the calibrated counters, where measured
+    tkey = {"fft_axis": "fft_axis_16f", "lie_pack": "lie_pack"}[dom] + "_fp64_" + "x".join(map(str, dims))
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    traffic = json.load(open(tfile)).get(tkey) if os.path.exists(tfile) else None
     # CPU: the oracle (GLU's 36-field transform, one thread) on a 16^3x32
     # sample of the same fixed field's shape, scaled by the volume
     cpu = None
@@ -393,7 +397,7 @@ def glueprop_main(a):
                    "cut": "psq", "max_mom": a.max_mom, "momenta": int(len(G))},
         "p50_ms": round(1e3 * pct(lat, 0.5), 3), "pspace_landau_error": err,
         "roofline": {"kernel": dom, "bound": "hbm", "achieved": round(ach, 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic},
         "kernels": kern, "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
