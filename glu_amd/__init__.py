@@ -24,7 +24,7 @@ DP = C.POINTER(C.c_double)
 _lib = None
 
 EXPORTS = [
-    "Landau", "Coulomb", "FACG", "FASD", "Coulomb_FA",
+    "Landau", "Coulomb", "FACG", "FASD", "Coulomb_FA", "cuts_struct",
     "glu_gfix_create", "glu_gfix_destroy", "glu_gfix_set_links", "glu_gfix_get_links",
     "glu_gfix_set_links_device", "glu_gfix_snapshot", "glu_gfix_restore", "glu_gfix_run",
     "glu_gfix_trace", "glu_gfix_measure", "glu_gfix_theta", "glu_gfix_reunit",

@@ -506,3 +506,65 @@ extern "C" size_t Coulomb(struct site *lat, const double accuracy, const size_t 
           Latt.gf_alpha, iters, tl, sl, plaq);
   return iters;
 }
+
+namespace {
+
+// the file name of Cuts/cut_output.c:115-258 for the momentum-space modes
+// built here (SIN_MOM is GLU's default, Headers/GLU_definitions.h:348-352)
+std::string cut_file(const struct cut_info &ci) {
+  char tmp[256];
+  std::string s(ci.where, strnlen(ci.where, sizeof(ci.where)));
+  s += ci.dir == GLUON_PROPS ? "G2F2" : "MOMgg";
+  snprintf(tmp, sizeof(tmp), "_SU3_%zux%zux%zux%zu_%zu_", Latt.dims[0], Latt.dims[1], Latt.dims[2], Latt.dims[3],
+           ci.max_mom);
+  s += tmp;
+  switch (ci.type) {
+    case HYPERCUBIC_CUT: s += "cutHYP"; break;
+    case PSQ_CUT: s += "cutPSQ"; break;
+    case CYLINDER_CUT: snprintf(tmp, sizeof(tmp), "cutCYL_w%g", ci.cyl_width); s += tmp; break;
+    case CYLINDER_AND_CONICAL_CUT: snprintf(tmp, sizeof(tmp), "cutCON_%zu", ci.angle); s += tmp; break;
+  }
+  snprintf(tmp, sizeof(tmp), "_SINMOM_LIN.%zu.bin", Latt.flow);
+  return s + tmp;
+}
+
+}  // namespace
+
+// Cuts/cuts.c:114-184 for GLUON_PROPS and EXCEPTIONAL with the LINEAR
+// definition; the transform, momentum list and contraction run on the device
+extern "C" int cuts_struct(struct site *A, const struct cut_info CUTINFO) {
+  const int kFailure = -1, kSuccess = 0;  // GLU_FAILURE, GLU_SUCCESS
+  if (CUTINFO.dir != GLUON_PROPS && CUTINFO.dir != EXCEPTIONAL) {
+    fprintf(stderr, "[CUTS] cut mode %d is not provided by this library\n", (int)CUTINFO.dir);
+    return kFailure;
+  }
+  if (CUTINFO.definition != LINEAR_DEF) {
+    fprintf(stderr, "[CUTS] only the LINEAR field definition is provided by this library\n");
+    return kFailure;
+  }
+  if ((int)CUTINFO.type < HYPERCUBIC_CUT || (int)CUTINFO.type > CYLINDER_AND_CONICAL_CUT) {
+    fprintf(stderr, "I don't recognise the type.. Leaving \n");
+    return kFailure;
+  }
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
+  if (!c) return kFailure;
+  std::vector<double> h;
+  pack(A, h);
+  const std::string path = cut_file(CUTINFO);
+  fprintf(stdout, "[CUTS] Our file is located @ %s \n", CUTINFO.where);
+  size_t nmom = 0;
+  const double angle = (double)CUTINFO.angle;
+  int rc = glu_gfix_set_links(c, h.data());
+  if (!rc)
+    rc = CUTINFO.dir == GLUON_PROPS
+             ? glu_gfix_gluon_props(c, (int)CUTINFO.type, CUTINFO.max_mom, CUTINFO.cyl_width, angle, path.c_str(),
+                                    &nmom, nullptr, nullptr, nullptr, 0, nullptr)
+             : glu_gfix_gluon_exceptional(c, (int)CUTINFO.type, CUTINFO.max_mom, CUTINFO.cyl_width, angle,
+                                          path.c_str(), &nmom, nullptr, nullptr, nullptr, 0, nullptr);
+  if (rc) {
+    fprintf(stderr, "[CUTS] device cuts failed: %s\n", glu_gfix_last_error());
+    return kFailure;
+  }
+  fprintf(stdout, "[CUTS] Writing finished...\n[CUTS] Outputting to %s\n", path.c_str());
+  return kSuccess;
+}

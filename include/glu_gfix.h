@@ -34,6 +34,14 @@ extern "C" {
 /* ------------------------------------------------------------------ */
 #ifndef GLU_ENUM_H
 typedef enum { GLU_FALSE, GLU_TRUE } GLU_bool; /* Headers/GLU_enum.h:83-85 */
+/* Headers/GLU_enum.h:159-196 */
+typedef enum {
+  EXCEPTIONAL, NONEXCEPTIONAL, FIELDS, SMEARED_GLUONS, INSTANTANEOUS_GLUONS, CONFIGSPACE_GLUONS,
+  GLUON_PROPS, STATIC_POTENTIAL, TOPOLOGICAL_CORRELATOR, TOPOLOGICAL_MOMENTS, TOPOLOGICAL_SUSCEPTIBILITY,
+  GLU_NOCUTS
+} cut_mode;
+typedef enum { LINEAR_DEF, LOG_DEF } lie_field_def;
+typedef enum { HYPERCUBIC_CUT, PSQ_CUT, CYLINDER_CUT, CYLINDER_AND_CONICAL_CUT } momentum_cut_def;
 #endif
 #ifndef GLU_TYPES_H
 /* Headers/GLU_types.h:269-273; O[mu] points at 9 complex doubles */
@@ -59,6 +67,17 @@ struct latt_info {
   int argc;
 };
 struct fftw_stuff; /* unused by this backend; callers may pass NULL */
+/* Headers/GLU_types.h:94-103 */
+struct cut_info {
+  cut_mode dir;
+  momentum_cut_def type;
+  size_t max_mom;
+  size_t max_t;
+  char where[256];
+  lie_field_def definition;
+  size_t angle;
+  double cyl_width;
+};
 #endif
 
 /* Landau gauge fixing with restarts and GLU's [GF] report lines.
@@ -169,6 +188,15 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 
 /* Reunitarise every device link (Geometry/random_config.c:43-56). */
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
+
+/* GLU's momentum-space cuts of gauge-fixed links, replacing
+   Cuts/cuts.c:114-184 (prototype Headers/cuts.h:41) for dir = GLUON_PROPS
+   and EXCEPTIONAL (GLU's default) with the LINEAR definition: the file
+   CUTINFO.where + GLU's name (Cuts/cut_output.c:115-258, from Latt.dims and
+   Latt.flow) receives the momentum list and G/F or g2/g3.  Other modes
+   and the LOG definition return GLU_FAILURE.  Unlike GLU, the links of A
+   are left as they were (GLU overwrites them with the Lie fields). */
+int cuts_struct(struct site *A, const struct cut_info CUTINFO);
 
 /* Transverse G(p^2) and longitudinal F(p^2) gluon propagators of the device
    links: GLU's cuts_struct with dir = GLUON_PROPS (Cuts/cuts.c:114-184),
