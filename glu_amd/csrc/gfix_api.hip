@@ -34,7 +34,6 @@ constexpr double kCgTol = 5e-11;       // CG_TOL, Headers/GLU_definitions.h:449-
 // mixed precision: lattices up to this many sites stop the FP32 phase at
 // its first stall (theta 30x above its minimum once below 1e-6)
 constexpr long long kStallSites = 1 << 17;
-constexpr long long kBatch = 8;        // iterations enqueued between host checks
 constexpr int kGluFailures = 7;        // GF_GLU_FAILURES, Headers/GLU_definitions.h:400-402
 
 int factor_radices(int N, int *rad) {
@@ -139,6 +138,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     CK(hipEventCreateWithFlags(&c->run_ev, hipEventDisableTiming));
     if (const char *e = getenv("GLU_GFIX_GRAPH")) c->use_graph = atoi(e);
+    if (const char *e = getenv("GLU_GFIX_BATCH")) c->batch = std::max(1, std::min(64, atoi(e)));
     CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->U64b, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
@@ -386,23 +386,30 @@ template <typename T> static Bufs bufs_for(glu_gfix_ctx *c) {
 template <typename T> static constexpr int tix() { return sizeof(T) == 4 ? 1 : 0; }
 
 // the CG gauge update, out of place and fused with the next derivative;
-// the host's link pointer moves to the buffer it writes
-template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
-  void *&cur = sizeof(T) == 4 ? c->U32 : c->U64;
-  void *&nxt = sizeof(T) == 4 ? c->U32b : c->U64b;
-  const Bufs b = bufs_for<T>(c);
-  GFIX_PROF(c, pk<T>(PK_EGDERIV),
-            launch_egauge_deriv<T>(cur, nxt, b.sn, b.G, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl,
-                                   c->stream));
-  std::swap(cur, nxt);
+// the host's link pointer moves to the buffer it writes.  The derivative
+// goes into the field buffer of in_old, and the two field pointers swap:
+// the `in` of this iteration becomes the in_old of the next (GLU's copy
+// old = in, Gfix/FACG.c:320-322, without moving the data)
+template <typename T> static void swap_cg_buffers(glu_gfix_ctx *c) {
+  std::swap(sizeof(T) == 4 ? c->U32 : c->U64, sizeof(T) == 4 ? c->U32b : c->U64b);
+  std::swap(sizeof(T) == 4 ? c->F32 : c->F, sizeof(T) == 4 ? c->old32 : c->old);
   c->upar[tix<T>()] ^= 1;
+  c->fpar[tix<T>()] ^= 1;
+}
+template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
+  const Bufs b = bufs_for<T>(c);
+  void *nxt = sizeof(T) == 4 ? c->U32b : c->U64b;
+  GFIX_PROF(c, pk<T>(PK_EGDERIV),
+            launch_egauge_deriv<T>(b.U, nxt, b.sn, b.G, b.F, b.old, c->part, geo_order(c, c->ord_eg), c->ctrl,
+                                   c->stream));
+  swap_cg_buffers<T>(c);
   c->deriv_ready[tix<T>()] = 1;
 }
 
 // SD start of steep_Landau_FACG, Gfix/FACG.c:249-265.  sn = in_old = in and
-// inold = <in, in> depend on `in` only, so they are taken before the update
-// (which overwrites `in` with the next derivative) and the update reads the
-// direction from the copy in sn.
+// inold = <in, in> depend on `in` only: sn is copied before the update,
+// which reads the direction from it, and `in` becomes in_old when the
+// update's field swap puts the next derivative in its place.
 template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   hipStream_t s = c->stream;
   launch_begin_segment(c->ctrl, 1, s);
@@ -411,7 +418,7 @@ template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   launch_line_search<T>(b.U, b.F, c->part, geo_order(c, c->ord_ls), c->ctrl, 1, s);
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 1, 0.15, 0.30, s);
-  launch_copy3<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s);
+  launch_copy<T>(b.F, b.sn, c->g.V, c->ctrl, s);
   launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
   launch_fin_sd(c->part, c->nblk, c->ctrl, s);
   enq_update_deriv<T>(c);
@@ -431,7 +438,7 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
   launch_fin_pr(c->part, c->nblk, c->ctrl, s);
-  GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, b.old, c->g.V, c->ctrl, s));
+  GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, c->g.V, c->ctrl, s));
   GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, geo_order(c, c->ord_ls), c->ctrl, 0, s));
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
   enq_update_deriv<T>(c);
@@ -521,7 +528,7 @@ static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, siz
 }
 
 template <typename T> static void enq_iterations(glu_gfix_ctx *c) {
-  for (long long k = 0; k < kBatch; k++) {
+  for (int k = 0; k < c->batch; k++) {
     if (c->run.method == GLU_GFIX_CG)
       enq_cg_iter<T>(c, bufs_for<T>(c));
     else
@@ -537,23 +544,26 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
   }
   // a pending derivative is enqueued outside the graph, so every captured
   // batch starts and ends with it in place (and, 8 swaps later, on the
-  // same link buffer it started on)
+  // same link and field buffers it started on)
   if (c->run.method == GLU_GFIX_CG && !c->deriv_ready[tix<T>()]) {
     const Bufs b = bufs_for<T>(c);
     launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, c->stream);
     launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), c->stream);
     c->deriv_ready[tix<T>()] = 1;
   }
-  const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1) + 4 * c->upar[tix<T>()];
+  const int kind = (sizeof(T) == 4 ? 2 : 0) + (c->run.method == GLU_GFIX_CG ? 0 : 1) + 4 * c->upar[tix<T>()] +
+                   8 * c->fpar[tix<T>()];
   if (!c->graph[kind]) {
     hipGraph_t gr = nullptr;
     if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
       return gfix_fail(-2, "graph capture failed to start");
-    enq_iterations<T>(c);
+    enq_iterations<T>(c);  // moves the host's buffer pointers as the batch will
     if (hipStreamEndCapture(c->stream, &gr) != hipSuccess || !gr) return gfix_fail(-2, "graph capture failed");
     const hipError_t e = hipGraphInstantiate(&c->graph[kind], gr, nullptr, nullptr, 0);
     hipGraphDestroy(gr);
     if (e != hipSuccess) return gfix_fail(-2, "graph instantiation failed: %s", hipGetErrorString(e));
+  } else if (c->run.method == GLU_GFIX_CG && (c->batch & 1)) {
+    swap_cg_buffers<T>(c);  // a replayed batch of odd length ends on the other buffers
   }
   return hipGraphLaunch(c->graph[kind], c->stream) == hipSuccess ? 0 : gfix_fail(-2, "graph launch failed");
 }

@@ -46,6 +46,9 @@ struct glu_gfix_ctx {
   // the swaps mod 2 (FP64, FP32) so captured graphs know their start
   void *U64b = nullptr, *U32b = nullptr;
   int upar[2] = {0, 0};
+  // the fused update also swaps the field buffers F <-> old (F32 <-> old32);
+  // fpar counts those swaps mod 2
+  int fpar[2] = {0, 0};
   // the fused update left the derivative of the current links (fields and
   // theta partials) in place: the next CG iteration needs no k_deriv
   int deriv_ready[2] = {0, 0};
@@ -66,10 +69,14 @@ struct glu_gfix_ctx {
   int ord_eg = 1, ord_ls = 1;
   int ord_cl = 1;  // Coulomb slice stencils: XCD-contiguous (1) or plain (0)
   int use_graph = 0;  // replay iteration batches from captured HIP graphs
-  hipGraphExec_t graph[8] = {};  // CG/SD x FP64/FP32 x link-buffer parity
+  // iterations enqueued between host checks; the iterations enqueued past
+  // a stop run as no-ops, so the result does not depend on it
+  int batch = 8;
+  hipGraphExec_t graph[16] = {};  // CG/SD x FP64/FP32 x link- and field-buffer parity
   hipEvent_t run_ev = nullptr;
   RunState run;
-  // packed fields (4 x V complex each): in, sn (search direction), in_old
+  // packed fields (4 x V complex each): in, sn (search direction), in_old;
+  // F and old trade places with every CG gauge update
   void *F = nullptr, *sn = nullptr, *old = nullptr;
   double *part = nullptr;  // per-block partial sums
   long long nblk = 0;

@@ -49,17 +49,17 @@ void launch_deriv(const void *U, void *F, double *part, const Geo &g, const Ctrl
 template <typename T>
 void launch_pr_sums(const void *F, const void *O, double *part, long long V, const Ctrl *ctrl, hipStream_t s);
 template <typename T>
-void launch_cg_update(const void *F, void *sn, void *O, long long V, const Ctrl *ctrl, hipStream_t s);
+void launch_cg_update(const void *F, void *sn, long long V, const Ctrl *ctrl, hipStream_t s);
 template <typename T>
-void launch_copy3(const void *F, void *a, void *b, long long V, const Ctrl *ctrl, hipStream_t s);
+void launch_copy(const void *F, void *a, long long V, const Ctrl *ctrl, hipStream_t s);
 template <typename T>
 void launch_line_search(const void *U, const void *F, double *part, const Geo &g, const Ctrl *ctrl, int force,
                         hipStream_t s);
 template <typename T>
 void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s);
 template <typename T>
-void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fout, double *part, const Geo &g,
-                         const Ctrl *ctrl, hipStream_t s);
+void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fin, void *Fout, double *part,
+                         const Geo &g, const Ctrl *ctrl, hipStream_t s);
 template <typename T>
 void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s);
 template <typename T>

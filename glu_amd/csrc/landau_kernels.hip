@@ -94,30 +94,26 @@ __global__ __launch_bounds__(kBlock) void k_pr_sums(const vec2<T> *__restrict__ 
   block_sum_store<2>(v, part + 2 * blockIdx.x);
 }
 
-// sn = in + beta sn ; old = in   (Gfix/FACG.c:315-323)
+// sn = in + beta sn (Gfix/FACG.c:315-323).  GLU's old = in needs no copy:
+// the fused update writes the next derivative into the field buffer that
+// held in_old, and the host swaps the two, so `in` becomes in_old in place.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cg_update(const vec2<T> *__restrict__ F, vec2<T> *__restrict__ sn,
-                                                     vec2<T> *__restrict__ O, long long n4,
-                                                     const Ctrl *__restrict__ ctrl) {
+                                                     long long n4, const Ctrl *__restrict__ ctrl) {
   if (ctrl->done) return;
   const T beta = (T)ctrl->beta;
   for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += (long long)gridDim.x * blockDim.x) {
     const cx<T> a = ld<T>(F + j), b = ld<T>(sn + j);
     st<T>(sn + j, mk<T>(a.re + beta * b.re, a.im + beta * b.im));
-    st<T>(O + j, a);
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_copy3(const vec2<T> *__restrict__ F, vec2<T> *__restrict__ a,
-                                                 vec2<T> *__restrict__ b, long long n4,
-                                                 const Ctrl *__restrict__ ctrl) {
+__global__ __launch_bounds__(kBlock) void k_copy(const vec2<T> *__restrict__ F, vec2<T> *__restrict__ a, long long n4,
+                                                const Ctrl *__restrict__ ctrl) {
   if (ctrl->done) return;
-  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += (long long)gridDim.x * blockDim.x) {
-    const vec2<T> v = F[j];
-    a[j] = v;
-    b[j] = v;
-  }
+  for (long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x; j < n4; j += (long long)gridDim.x * blockDim.x)
+    a[j] = F[j];
 }
 
 template <typename T>
@@ -225,22 +221,32 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
 // the arithmetic of the site that stores them, and the old links of the
 // neighbours must stay readable meanwhile.  The derivative of the updated
 // links then costs no second pass over them.  D is the search direction;
-// Fout receives the packed derivative and part its theta partials.  Past
-// convergence the links are copied unchanged, so the buffer swap the host
-// made at enqueue time holds whichever iteration stopped.
+// Fout (the buffer of in_old, which this iteration no longer needs)
+// receives the packed derivative and part its theta partials; the host
+// then swaps the link buffers and the field buffers Fin <-> Fout.  Past
+// convergence the links are copied unchanged and Fin and Fout exchange
+// their contents, so the swaps the host made at enqueue time hold
+// whichever iteration stopped, with every buffer in its role.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_egauge_deriv(const vec2<T> *__restrict__ Uin, vec2<T> *__restrict__ Uout,
                                                         const vec2<T> *__restrict__ D, double2 *__restrict__ G,
-                                                        vec2<T> *__restrict__ Fout, double *__restrict__ part, Geo g,
-                                                        const Ctrl *__restrict__ ctrl) {
+                                                        vec2<T> *__restrict__ Fin, vec2<T> *__restrict__ Fout,
+                                                        double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
   const long long V = g.V;
   const unsigned lb = logical_block(g);
   const long long i = (long long)lb * blockDim.x + threadIdx.x;
   if (ctrl->done) {
-    if (i < V)
+    if (i < V) {
 #pragma unroll 4
       for (int k = 0; k < kND * 6; k++) Uout[(long long)k * V + i] = Uin[(long long)k * V + i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const vec2<T> a = Fin[(long long)k * V + i], b = Fout[(long long)k * V + i];
+        Fin[(long long)k * V + i] = b;
+        Fout[(long long)k * V + i] = a;
+      }
+    }
     return;
   }
   double loc[1] = {0.0};
@@ -608,18 +614,15 @@ void launch_pr_sums(const void *F, const void *O, double *part, long long V, con
                      V, ctrl);
 }
 template <typename T>
-void launch_cg_update(const void *F, void *sn, void *O, long long V, const Ctrl *ctrl, hipStream_t s) {
+void launch_cg_update(const void *F, void *sn, long long V, const Ctrl *ctrl, hipStream_t s) {
   const long long n4 = 4 * V;
   const unsigned b = (unsigned)std::min<long long>(nblk(n4), 8192);
-  hipLaunchKernelGGL(k_cg_update<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)sn, (vec2<T> *)O,
-                     n4, ctrl);
+  hipLaunchKernelGGL(k_cg_update<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)sn, n4, ctrl);
 }
-template <typename T>
-void launch_copy3(const void *F, void *a, void *b2, long long V, const Ctrl *ctrl, hipStream_t s) {
+template <typename T> void launch_copy(const void *F, void *a, long long V, const Ctrl *ctrl, hipStream_t s) {
   const long long n4 = 4 * V;
   const unsigned b = (unsigned)std::min<long long>(nblk(n4), 8192);
-  hipLaunchKernelGGL(k_copy3<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)a, (vec2<T> *)b2, n4,
-                     ctrl);
+  hipLaunchKernelGGL(k_copy<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)a, n4, ctrl);
 }
 template <typename T>
 void launch_line_search(const void *U, const void *F, double *part, const Geo &g, const Ctrl *ctrl, int force,
@@ -633,10 +636,10 @@ void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ct
                      (double2 *)G, g, ctrl, fixed);
 }
 template <typename T>
-void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fout, double *part, const Geo &g,
-                         const Ctrl *ctrl, hipStream_t s) {
+void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fin, void *Fout, double *part,
+                         const Geo &g, const Ctrl *ctrl, hipStream_t s) {
   hipLaunchKernelGGL(k_egauge_deriv<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)Uin, (vec2<T> *)Uout,
-                     (const vec2<T> *)D, (double2 *)G, (vec2<T> *)Fout, part, g, ctrl);
+                     (const vec2<T> *)D, (double2 *)G, (vec2<T> *)Fin, (vec2<T> *)Fout, part, g, ctrl);
 }
 template <typename T>
 void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s) {
@@ -653,13 +656,13 @@ template <typename T> void launch_identity(void *G, long long V, hipStream_t s) 
 #define GFIX_INST(T)                                                                                          \
   template void launch_deriv<T>(const void *, void *, double *, const Geo &, const Ctrl *, hipStream_t);      \
   template void launch_pr_sums<T>(const void *, const void *, double *, long long, const Ctrl *, hipStream_t); \
-  template void launch_cg_update<T>(const void *, void *, void *, long long, const Ctrl *, hipStream_t);       \
-  template void launch_copy3<T>(const void *, void *, void *, long long, const Ctrl *, hipStream_t);           \
+  template void launch_cg_update<T>(const void *, void *, long long, const Ctrl *, hipStream_t);               \
+  template void launch_copy<T>(const void *, void *, long long, const Ctrl *, hipStream_t);                    \
   template void launch_line_search<T>(const void *, const void *, double *, const Geo &, const Ctrl *, int,   \
                                       hipStream_t);                                                           \
   template void launch_egauge<T>(void *, const void *, void *, const Geo &, const Ctrl *, int, hipStream_t);  \
-  template void launch_egauge_deriv<T>(const void *, void *, const void *, void *, void *, double *, const Geo &, \
-                                       const Ctrl *, hipStream_t);                                                \
+  template void launch_egauge_deriv<T>(const void *, void *, const void *, void *, void *, void *, double *,     \
+                                       const Geo &, const Ctrl *, hipStream_t);                                   \
   template void launch_pack<T>(const double *, void *, long long, int, long long, long long, hipStream_t);     \
   template void launch_unpack<T>(const void *, double *, long long, int, long long, long long, hipStream_t);   \
   template void launch_identity<T>(void *, long long, hipStream_t);

@@ -106,13 +106,14 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
       hipMemcpyAsync(dc, &h, sizeof(Ctrl), hipMemcpyHostToDevice, c->stream) == hipSuccess) {
     void *Unext = sizeof(T) == 4 ? c->U32b : c->U64b;
     void *sn = sizeof(T) == 4 ? c->sn32 : c->sn;
+    void *Fold = sizeof(T) == 4 ? c->old32 : c->old;
     void *G = sizeof(T) == 4 ? c->G64 : nullptr;
     const int norder = geo_slab_order_ok(c->g) ? 3 : 2;
     const char *names[3] = {"plain", "XCD-contiguous", "XCD t-slabs"};
     float eg[3] = {1e30f, 1e30f, 1e30f}, ls[3] = {1e30f, 1e30f, 1e30f};
     for (int o = 0; o < norder; o++) {
       const Geo g = geo_order(c, o);
-      eg[o] = time_ms(c, [&] { launch_egauge_deriv<T>(U, Unext, sn, G, F, c->part, g, dc, c->stream); }, kReps);
+      eg[o] = time_ms(c, [&] { launch_egauge_deriv<T>(U, Unext, sn, G, Fold, F, c->part, g, dc, c->stream); }, kReps);
       ls[o] = time_ms(c, [&] { launch_line_search<T>(U, sn, c->part, g, dc, 1, c->stream); }, kReps);
       snprintf(line, sizeof(line), "order %s: gauge update + derivative %.4f ms, line search %.4f ms\n", names[o],
                eg[o], ls[o]);

@@ -223,6 +223,31 @@ def test_mixed_stall_exit(stall, monkeypatch):
     assert abs(m["plaquette"] - plaq) < 1e-14
 
 
+@pytest.mark.parametrize("graph", ["0", "1"])
+def test_batch_length_is_bitwise_neutral(graph, monkeypatch):
+    """Iterations enqueued past a stop run as no-ops that keep the links and
+    the CG fields (in, sn, in_old) in their buffers, so a solve does not
+    depend on how many iterations the host enqueues between checks, with
+    or without graph replay.  The mixed solve carries its CG state over
+    the FP32 -> FP64 switch when the two thetas there agree, so a misplaced
+    in_old would change its trajectory."""
+    dims = (8, 8, 8, 16)
+    U, _ = heatbath_config(dims, sweeps=10)
+    monkeypatch.setenv("GLU_GFIX_GRAPH", graph)
+    monkeypatch.setenv("GLU_GFIX_STALL_FACTOR", "0")  # hand over at the switch point, CG state kept
+    out = []
+    for batch in ("1", "3", "8"):
+        monkeypatch.setenv("GLU_GFIX_BATCH", batch)
+        print(f"batch {batch} graph {graph}", flush=True)
+        with fresh(dims, U, gl.MIXED) as g:
+            it, th = g.run(gl.CG, 0.08, 1e-14, 1000)
+            out.append((it, th, g.mixed_split(), g.trace(), g.get_links()))
+    for o in out[1:]:
+        assert out[0][:3] == o[:3]
+        assert np.array_equal(out[0][3], o[3])
+        assert np.array_equal(out[0][4], o[4])
+
+
 def test_unit_and_pure_gauge():
     dims = (4, 4, 4, 8)
     V = int(np.prod(dims))
