@@ -74,7 +74,7 @@ size_t gfix_footprint(const size_t dims[4], int gftype, int precision) {
   size_t b = 2 * V * 4 * 6 * sizeof(double2);  // FP64 links, current and next
   b += 3 * V * 4 * sizeof(double2);           // in, sn, in_old
   if (precision == GLU_GFIX_MIXED)
-    b += 2 * V * 4 * 6 * sizeof(float2) + V * 6 * sizeof(float2) + 3 * V * 4 * sizeof(float2);
+    b += 2 * V * 4 * 6 * sizeof(float2) + V * 6 * sizeof(double2) + 3 * V * 4 * sizeof(float2);
   if (gftype == GLU_GFIX_COULOMB) b += 2 * V * 6 * sizeof(double2);  // slice transforms + scratch
   b += ((V + kBlock - 1) / kBlock) * 4 * sizeof(double) + (64 << 10);
   return b;
@@ -151,7 +151,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     if (precision == GLU_GFIX_MIXED) {
       CK(hipMalloc(&c->U32, (size_t)V * 24 * sizeof(float2)));
       CK(hipMalloc(&c->U32b, (size_t)V * 24 * sizeof(float2)));
-      CK(hipMalloc(&c->G32, (size_t)V * 6 * sizeof(float2)));
+      CK(hipMalloc(&c->G64, (size_t)V * 6 * sizeof(double2)));
       CK(hipMalloc(&c->F32, (size_t)V * 4 * sizeof(float2)));
       CK(hipMalloc(&c->sn32, (size_t)V * 4 * sizeof(float2)));
       CK(hipMalloc(&c->old32, (size_t)V * 4 * sizeof(float2)));
@@ -215,7 +215,7 @@ bad:
 extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
   if (c->stream) hipStreamSynchronize(c->stream);
-  void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G32, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
+  void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
                   c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->lie, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -378,7 +378,7 @@ struct Bufs {
   void *U, *G, *F, *sn, *old;
 };
 template <typename T> static Bufs bufs_for(glu_gfix_ctx *c) {
-  if (sizeof(T) == 4) return {c->U32, c->G32, c->F32, c->sn32, c->old32};
+  if (sizeof(T) == 4) return {c->U32, c->G64, c->F32, c->sn32, c->old32};
   return {c->U64, nullptr, c->F, c->sn, c->old};
 }
 
@@ -610,9 +610,9 @@ int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_
   }
   r.mixed = c->precision == GLU_GFIX_MIXED && c->switch_theta > acc;
   if (r.mixed) {
-    // FP32 phase on a demoted copy, with the site transform accumulated
+    // FP32 phase on a demoted copy; the site transform accumulates in FP64
     launch_demote(c->U64, c->U32, c->g.V * 24, c->stream);
-    launch_identity<float>(c->G32, c->g.V, c->stream);
+    launch_identity<double>(c->G64, c->g.V, c->stream);
     r.phase = PH_FP32;
     if (int rc = phase_ctrl_init(c, 0, c->switch_theta, max_iters)) return rc;
   } else {
@@ -651,7 +651,7 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     // FP32 -> FP64: replay the accumulated transform on the FP64 links and
     // carry the CG state when the links agree with where FP32 stopped
     c->last_fp32_iters = (size_t)h.total;
-    launch_apply_gauge(c->U64, c->G32, c->g, c->stream);
+    launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
     launch_promote(c->sn32, c->sn, c->g.V * 4, c->stream);
     launch_promote(c->old32, c->old, c->g.V * 4, c->stream);
     double thl = 0.0;
@@ -680,7 +680,7 @@ int landau_advance(glu_gfix_ctx *c, int block) {
   }
   if (r.phase == PH_FP32) {  // mixed run that stopped in FP32 (converged or failed)
     c->last_fp32_iters = (size_t)h.total;
-    launch_apply_gauge(c->U64, c->G32, c->g, c->stream);
+    launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "sync failed");
     r.iters = out;
   } else {

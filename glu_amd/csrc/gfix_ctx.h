@@ -38,7 +38,7 @@ struct glu_gfix_ctx {
   hipStream_t stream = nullptr;
   // links: FP64 column planes (4 dirs x 6 entries x V), FP32 copy and the
   // FP32 accumulated site transform for the mixed schedule, snapshot
-  void *U64 = nullptr, *U32 = nullptr, *G32 = nullptr, *Usnap = nullptr;
+  void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
   void *Uorig = nullptr;  // input links kept for the Landau() restarts
   void *Ubest = nullptr;  // best Gribov copy so far (glu_gfix_luxury)
   // second link buffers: the CG gauge update writes out of place and the

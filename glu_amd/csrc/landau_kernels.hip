@@ -180,23 +180,14 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
 
 // ---------------------------------------------------------------------------
 // U_mu(x) <- g(x) U_mu(x) g(x+mu)^dagger: Gfix/line_search.c:60-97.
-// With G != null the site transform is also accumulated,
-// G(x) <- g(x) G(x), in the storage precision and kept on SU(3) like the
-// links: the mixed schedule keeps the FP64 links untouched during the FP32
-// phase and applies G to them at the switch, reunitarised in FP64, so the
-// FP64 links only ever receive an exactly unitary transform.
+// With G != null the site transform is also accumulated in FP64,
+// G(x) <- g(x) G(x) with g rebuilt in FP64 from the same field values: the
+// mixed schedule keeps the FP64 links untouched during the FP32 phase and
+// replays G on them exactly at the switch.
 // ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ void accumulate_transform(vec2<T> *__restrict__ G, long long V, long long i,
-                                                     const su3<T> &A) {
-  su3<T> n = mul_cols<T>(A, load_link<T>(G, 0, i, V));
-  if (sizeof(T) == 4) gram_reunit<T>(n);
-  store_link<T>(G, 0, i, V, n);
-}
-
-template <typename T>
 __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, const vec2<T> *__restrict__ F,
-                                                  vec2<T> *__restrict__ G, Geo g, const Ctrl *__restrict__ ctrl,
+                                                  double2 *__restrict__ G, Geo g, const Ctrl *__restrict__ ctrl,
                                                   int fixed_step) {
   if (ctrl->done) return;
   const long long V = g.V;
@@ -216,7 +207,10 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
     if (sizeof(T) == 4) gram_reunit<T>(n);
     store_link<T>(U, mu, i, V, n);
   }
-  if (G) accumulate_transform<T>(G, V, i, A);
+  if (G) {
+    const su3<double> A64 = gauge_at64<T>(F, V, i, alpha);
+    store_link<double>(G, 0, i, V, mul_cols<double>(A64, load_link<double>(G, 0, i, V)));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -236,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_egauge_deriv(const vec2<T> *__restrict__ Uin, vec2<T> *__restrict__ Uout,
-                                                        const vec2<T> *__restrict__ D, vec2<T> *__restrict__ G,
+                                                        const vec2<T> *__restrict__ D, double2 *__restrict__ G,
                                                         vec2<T> *__restrict__ Fin, vec2<T> *__restrict__ Fout,
                                                         double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
   const long long V = g.V;
@@ -275,7 +269,10 @@ __global__ __launch_bounds__(kBlock) void k_egauge_deriv(const vec2<T> *__restri
       d.add(n, sb);
     }
     loc[0] = d.store(Fout, V, i);
-    if (G) accumulate_transform<T>(G, V, i, A);
+    if (G) {
+      const su3<double> A64 = gauge_at64<T>(D, V, i, alpha);
+      store_link<double>(G, 0, i, V, mul_cols<double>(A64, load_link<double>(G, 0, i, V)));
+    }
   }
   block_sum_store<1>(loc, part + lb);
 }
@@ -517,9 +514,9 @@ __global__ __launch_bounds__(kBlock) void k_identity(vec2<T> *__restrict__ G, lo
   store_link<T>(G, 0, i, V, m);
 }
 
-// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dagger in FP64 with the accumulated FP32
-// site transform promoted and reunitarised in FP64 first (Gfix/gtrans.c:169-189)
-__global__ __launch_bounds__(kBlock) void k_apply_gauge(double2 *__restrict__ U, const float2 *__restrict__ G,
+// U_mu(x) <- G(x) U_mu(x) G(x+mu)^dagger in FP64 with the accumulated site
+// transform reunitarised first (Gfix/gtrans.c:169-189)
+__global__ __launch_bounds__(kBlock) void k_apply_gauge(double2 *__restrict__ U, const double2 *__restrict__ G,
                                                        Geo g) {
   const long long V = g.V;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -527,7 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_gauge(double2 *__restrict__ U,
   int x[4];
   coords(g, i, x);
   auto fetch = [&](long long s) {
-    su3<double> d = promote<float>(load_link<float>(G, 0, s, V));
+    su3<double> d = load_link<double>(G, 0, s, V);
     gram_reunit<double>(d);
     return d;
   };
@@ -636,13 +633,13 @@ void launch_line_search(const void *U, const void *F, double *part, const Geo &g
 template <typename T>
 void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s) {
   hipLaunchKernelGGL(k_egauge<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (vec2<T> *)U, (const vec2<T> *)F,
-                     (vec2<T> *)G, g, ctrl, fixed);
+                     (double2 *)G, g, ctrl, fixed);
 }
 template <typename T>
 void launch_egauge_deriv(const void *Uin, void *Uout, const void *D, void *G, void *Fin, void *Fout, double *part,
                          const Geo &g, const Ctrl *ctrl, hipStream_t s) {
   hipLaunchKernelGGL(k_egauge_deriv<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)Uin, (vec2<T> *)Uout,
-                     (const vec2<T> *)D, (vec2<T> *)G, (vec2<T> *)Fin, (vec2<T> *)Fout, part, g, ctrl);
+                     (const vec2<T> *)D, (double2 *)G, (vec2<T> *)Fin, (vec2<T> *)Fout, part, g, ctrl);
 }
 template <typename T>
 void launch_pack(const double *aos, void *U, long long V, int nd, long long site0, long long n, hipStream_t s) {
@@ -699,7 +696,7 @@ void launch_demote(const void *U, void *W, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_demote, dim3(b), dim3(kBlock), 0, s, (const double2 *)U, (float2 *)W, n);
 }
 void launch_apply_gauge(void *U, const void *G, const Geo &g, hipStream_t s) {
-  hipLaunchKernelGGL(k_apply_gauge, dim3(nblk(g.V)), dim3(kBlock), 0, s, (double2 *)U, (const float2 *)G, g);
+  hipLaunchKernelGGL(k_apply_gauge, dim3(nblk(g.V)), dim3(kBlock), 0, s, (double2 *)U, (const double2 *)G, g);
 }
 void launch_promote(const void *a, void *b, long long n, hipStream_t s) {
   const unsigned bl = (unsigned)std::min<long long>(nblk(n), 16384);

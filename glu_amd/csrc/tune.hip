@@ -107,7 +107,7 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
     void *Unext = sizeof(T) == 4 ? c->U32b : c->U64b;
     void *sn = sizeof(T) == 4 ? c->sn32 : c->sn;
     void *Fold = sizeof(T) == 4 ? c->old32 : c->old;
-    void *G = sizeof(T) == 4 ? c->G32 : nullptr;
+    void *G = sizeof(T) == 4 ? c->G64 : nullptr;
     const int norder = geo_slab_order_ok(c->g) ? 3 : 2;
     const char *names[3] = {"plain", "XCD-contiguous", "XCD t-slabs"};
     float eg[3] = {1e30f, 1e30f, 1e30f}, ls[3] = {1e30f, 1e30f, 1e30f};
