@@ -37,7 +37,7 @@ struct glu_gfix_ctx {
   int precision = 0;
   hipStream_t stream = nullptr;
   // links: FP64 column planes (4 dirs x 6 entries x V), FP32 copy and the
-  // FP32 accumulated site transform for the mixed schedule, snapshot
+  // FP64 accumulated site transform for the mixed schedule, snapshot
   void *U64 = nullptr, *U32 = nullptr, *G64 = nullptr, *Usnap = nullptr;
   void *Uorig = nullptr;  // input links kept for the Landau() restarts
   void *Ubest = nullptr;  // best Gribov copy so far (glu_gfix_luxury)
