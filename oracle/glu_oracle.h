@@ -33,8 +33,9 @@
  * the loops by pure-gauge inputs and invariants; the gluon propagators
  * (tests/test_oracle_glueprop.py) by brute-force momentum lists, numpy and
  * a plane wave; the exceptional g2 by the propagators' trace, g3 by
- * colour-rotation invariance and numpy around GLU's contraction formula
- * (partly pinned).  Parity unpinned: the
+ * colour-rotation invariance and by GLU's alternative triple-sum
+ * contraction (TRACE_PROJECTION, Cuts/MOMgg.c:67-91) in numpy.  Parity
+ * unpinned: the
  * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
  * trajectories, and or_heatbath, which is an independent generator (a
  * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.

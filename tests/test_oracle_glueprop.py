@@ -218,8 +218,14 @@ def test_exceptional_is_colour_rotation_invariant():
 
 def test_exceptional_g3_against_numpy():
     """The field pipeline (lists, transform, Landau phase, sites of -p and
-    0) by numpy; the contraction is MOMgg.c's contraction_matrices written
-    out again, so this pins the pipeline around it, not the formula."""
+    0) by numpy, and the contraction by GLU's TRACE_PROJECTION form
+    (Cuts/MOMgg.c:67-91): the explicit triple sum
+    sum_{mu nu rho} (delta_mu nu - p_mu p_nu / p^2) p_rho
+    tr(A_mu(p) A_nu(-p) A_rho(0)) / p^2.  The oracle follows the default
+    contraction_matrices (MOMgg.c:93-125), which groups the same sum into
+    matrix products; the two agree by the cyclicity of the trace, so a
+    reversed product, a lost sign on the -p/p^2 term or A(p) and A(-p)
+    exchanged would show here."""
     dims = (4, 4, 4, 6)
     rng = np.random.default_rng(9)
     V = int(np.prod(dims))
@@ -241,9 +247,8 @@ def test_exceptional_g3_against_numpy():
         p = 2 * np.sin(np.pi * k / Lv)
         p2 = (p ** 2).sum()
         p2 = p2 if p2 >= 3e-14 else 1.0
-        AA = sum(Ap[m] @ Am[m] for m in range(4))
-        Arho = sum(p[m] * A0[m] for m in range(4))
-        AP = sum(p[m] * Ap[m] for m in range(4))
-        AM = sum(-p[m] / p2 * Am[m] for m in range(4))
-        want = ((np.trace(AA @ Arho) + np.trace(Arho @ AP @ AM)) / p2).real * 4 / (2 * 3 * 8 * 3 * V)
+        T = np.einsum("aij,bjk,cki->abc", Ap, Am, A0)  # tr(A_mu(p) A_nu(-p) A_rho(0))
+        proj = np.eye(4) - np.outer(p, p) / p2
+        want = (np.einsum("ab,c,abc->", proj, p, T) / p2).real * 4 / (2 * 3 * 8 * 3 * V)
         assert abs(g3[i] - want) < 1e-12 * max(np.abs(g3).max(), 1e-300)
+    assert np.abs(g3).max() > 1e-6
