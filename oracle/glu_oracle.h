@@ -34,9 +34,10 @@
  * (tests/test_oracle_glueprop.py) by brute-force momentum lists, numpy and
  * a plane wave; the exceptional g2 by the propagators' trace, g3 by
  * colour-rotation invariance and by GLU's alternative triple-sum
- * contraction (TRACE_PROJECTION, Cuts/MOMgg.c:67-91) in numpy.  Parity
- * unpinned: the
- * MWC_4096 stream values (statistics only), the exact FACG/Coulomb
+ * contraction (TRACE_PROJECTION, Cuts/MOMgg.c:67-91) in numpy; the
+ * MWC_4096 stream by big-integer restatements of MT19937's init_genrand
+ * (with its published state for seed 5489) and Marsaglia's complementary
+ * multiply-with-carry.  Parity unpinned: the exact FACG/Coulomb
  * trajectories, and or_heatbath, which is an independent generator (a
  * counter-based Cabibbo-Marinari heatbath), not a restatement of GLU's.
  */

@@ -21,6 +21,9 @@ sign or index, transposed operand) would fail:
                                        (tests/U_Nops_test.c:378-389 pattern)
   random gauge transform               plaquette/Polyakov-loop invariance
                                        (tests/units.c:210-239 pattern)
+  MWC_4096 stream                      big-integer MT19937 init_genrand
+                                       (published state for seed 5489) and
+                                       Marsaglia's CMWC by its definition
   line search probes                   equal the functional after a trial
                                        egauge at each probe step
   approx_minimum                       exact parabola vertex and the clamps
@@ -28,9 +31,8 @@ sign or index, transposed operand) would fail:
                                        theta < acc, plaquette preserved,
                                        functional decreasing
 
-Parity unpinned: the MWC_4096 stream values (only its statistics are
-checked) and the exact FACG trajectory (theta_k, alpha_k, beta_k), which
-only a run of the reference could pin.
+Parity unpinned: the exact FACG trajectory (theta_k, alpha_k, beta_k),
+which only a run of the reference could pin.
 """
 import os
 
@@ -331,6 +333,47 @@ def test_random_gtrans_invariance(conf):
     assert np.abs(G - U).max() > 0.1
     m = as_mats(G.reshape(-1, 18))
     assert np.abs(m @ np.conj(np.swapaxes(m, 1, 2)) - np.eye(3)).max() < 1e-14
+
+
+def mt_init_table(seed):
+    """MT19937's init_genrand recurrence (Matsumoto and Nishimura, 1998),
+    which GLU uses to fill its table (Utils/par_MWC_4096.c:72-89)"""
+    t = [seed & 0xFFFFFFFF]
+    for j in range(1, 4096):
+        t.append((1812433253 * (t[-1] ^ (t[-1] >> 30)) + j) & 0xFFFFFFFF)
+    return t
+
+
+def cmwc(x, c, n):
+    """Marsaglia's complementary multiply-with-carry with multiplier 18782
+    and base b = 2^32 - 1, by its definition in integers: t = a x + c,
+    c = t div b, x = (b - 1) - (t mod b).  GLU forms t mod b with a carry
+    trick on 32-bit halves, which agrees except when the two halves sum to
+    exactly b (odds 2^-32 per draw)."""
+    b = 2**32 - 1
+    out = []
+    for _ in range(n):
+        c, r = divmod(18782 * x + c, b)
+        x = (b - 1) - r
+        out.append(x)
+    return out, x, c
+
+
+def test_rng_stream_against_published_algorithms():
+    """The oracle's draws equal an independent big-integer restatement of
+    the two published algorithms GLU combines: the table seeded by MT19937's
+    init_genrand (its published state for seed 5489 has mt[1] = 1301868182),
+    the carry taken as table[4095] mod 809430660, and the lag-1 CMWC on
+    table[0] (GLU resets its index every call, par_MWC_4096.c:33-46), with
+    thread t seeded by SEED + t in 32 bits and warmed up by 10000 draws
+    (Utils/par_rng.c:73-120).  A draw is x 2^-32 (par_MWC_4096.c:93-112)."""
+    assert mt_init_table(5489)[1] == 1301868182
+    for seed, nth, thread in [(5, 3, 1), (1, 1, 0), (2**32 - 2, 2, 1), (2**32 - 1, 2, 1)]:
+        tab = mt_init_table((seed + thread) & 0xFFFFFFFF)
+        _, x, c = cmwc(tab[0], tab[4095] % 809430660, 10000)
+        want, _, _ = cmwc(x, c, 200)
+        got = O.rng_draws(seed, nth, thread, 200)
+        assert np.array_equal(got, np.array(want, dtype=np.float64) / 2.0**32), (seed, thread)
 
 
 def test_rng_statistics():
