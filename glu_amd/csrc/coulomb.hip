@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../../include/glu_gfix.h"
@@ -181,8 +182,12 @@ __global__ void k_cfin_pr(const double *__restrict__ part, long long nbs, SliceC
 }
 
 // sn = in + beta sn ; old = in  (mode 0: sn = old = in)
-__global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ F, double2 *__restrict__ sn,
-                                                   double2 *__restrict__ O, Geo g, SliceCtrl sc, int mode) {
+// sn = in + beta sn (mode 1) or sn = in (mode 0, the SD start).  GLU's
+// copy old = in is not made: the host swaps its in and in_old pointers
+// after every CG iteration, so the next derivative lands in the old buffer
+// and this iteration's in becomes in_old in place.
+__global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ F, double2 *__restrict__ sn, Geo g,
+                                                   SliceCtrl sc, int mode) {
   int t;
   long long bx;
   slice_block(g, t, bx);
@@ -199,7 +204,6 @@ __global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ 
     } else {
       sn[f * V + i] = a;
     }
-    O[f * V + i] = a;
   }
 }
 
@@ -459,7 +463,10 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
           *O = (double2 *)c->old;
   double *part = c->part;
   if (method == GLU_GFIX_CG) {
-    // SD start: derivative, FA, theta and inold, line search, sn = old = in
+    // SD start: derivative, FA, theta and inold, line search, sn = in; the
+    // pointer swap makes in the in_old of the first CG iteration.  The CG
+    // state lives within this call (every call starts with an SD step), so
+    // a slice that stopped may keep its fields in either buffer.
     hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc);
     enq_fa<double>(c, F, nullptr, sc.done);
     hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs);
@@ -467,7 +474,8 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, F, part, g, sc, cs->nbs, 1);
     hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 1, 0);
     hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc);
-    hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, O, g, sc, 0);
+    hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 0);
+    std::swap(F, O);
   }
   for (int round = 0;; round++) {
     for (int b = 0; b < 8; b++) {
@@ -476,10 +484,11 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
       if (method == GLU_GFIX_CG) {
         hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, O, part, g, sc, cs->nbs);
         hipLaunchKernelGGL(k_cfin_pr, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 1);
-        hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, O, g, sc, 1);
+        hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 1);
         hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, sn, part, g, sc, cs->nbs, 0);
         hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0, 0);
         hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, sn, g, sc);
+        std::swap(F, O);
       } else {
         hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0, 1);
         hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc);
