@@ -258,8 +258,9 @@ constexpr int fast_P(int N) {
   return p;
 }
 
-// element n of pencil j at lds[n * SN + j * SJ]
-template <typename T, int N, int P, int R, int Ns, int SN, int SJ>
+// element n of pencil j at lds[n * SN + (j % PJ) * SJ + (j / PJ) * PS]:
+// pencils come in groups of PJ (one group per plane of a multi-plane tile)
+template <typename T, int N, int P, int R, int Ns, int SN, int SJ, int PJ = P, int PS = 0>
 __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int sign) {
   constexpr int nbf = N / R, total = nbf * P, tstep = N / (Ns * R);
   constexpr int items = (total + kFftThreads - 1) / kFftThreads;
@@ -269,8 +270,9 @@ __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int s
     const int w = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads == 0 || w < total) {
       const int j = w % P, b = w / P, k = b % Ns;
+      const int jo = (j % PJ) * SJ + (j / PJ) * PS;
 #pragma unroll
-      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * SN + j * SJ);
+      for (int q = 0; q < R; q++) v[it][q] = ld<T>(lds + (b + q * nbf) * SN + jo);
       if constexpr (Ns > 1) {
 #pragma unroll
         for (int q = 1; q < R; q++) {
@@ -288,50 +290,57 @@ __device__ __forceinline__ void fast_stage(vec2<T> *lds, const cx<T> *twl, int s
     const int w = threadIdx.x + it * kFftThreads;
     if (total % kFftThreads == 0 || w < total) {
       const int j = w % P, b = w / P, k = b % Ns;
+      const int jo = (j % PJ) * SJ + (j / PJ) * PS;
       const int out = (b / Ns) * Ns * R + k;
 #pragma unroll
-      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * SN + j * SJ, v[it][q]);
+      for (int q = 0; q < R; q++) st<T>(lds + (out + q * Ns) * SN + jo, v[it][q]);
     }
   }
   __syncthreads();
 }
 
-template <typename T, int N, int P, int S_, int Ns, int SN = P + 1, int SJ = 1>
+template <typename T, int N, int P, int S_, int Ns, int SN = P + 1, int SJ = 1, int PJ = P, int PS = 0>
 __device__ __forceinline__ void fast_run(vec2<T> *lds, const cx<T> *twl, int sign) {
   constexpr RadPlan pl = make_plan(N);
   if constexpr (S_ < pl.n) {
     constexpr int R = pl.r[S_];
-    fast_stage<T, N, P, R, Ns, SN, SJ>(lds, twl, sign);
-    fast_run<T, N, P, S_ + 1, Ns * R, SN, SJ>(lds, twl, sign);
+    fast_stage<T, N, P, R, Ns, SN, SJ, PJ, PS>(lds, twl, sign);
+    fast_run<T, N, P, S_ + 1, Ns * R, SN, SJ, PJ, PS>(lds, twl, sign);
   }
 }
 
-// both in-plane axes of an x-y plane in one pass: the plane (contiguous in
-// memory) is staged as lds[x * (NY+1) + y]; the x transform runs on pencils
-// y with stride NY+1 between points, the y transform on pencils x with
-// stride 1; NY+1 is odd, so both access patterns are bank-conflict free
-template <typename T, int NX, int NY>
+// both in-plane axes of NPL consecutive x-y planes in one pass: the planes
+// (contiguous in memory) are staged as lds[x * (NPL NY + 1) + p NY + y]; the
+// x transform runs on the NPL NY pencils (p, y) with stride NPL NY + 1
+// between points, the y transform on the NPL NX pencils (p, x) with stride
+// 1; the row length NPL NY + 1 is odd, so both access patterns are
+// bank-conflict free.  With two 32 x 32 planes per workgroup every thread
+// is busy in every stage (one plane leaves half of them idle in the
+// radix-8 stages).
+template <typename T, int NX, int NY, int NPL>
 __global__ __launch_bounds__(kFftThreads) void k_fft_xy(vec2<T> *__restrict__ F, long long fstride,
                                                        const double2 *__restrict__ twx,
                                                        const double2 *__restrict__ twy, int mode,
                                                        const Ctrl *__restrict__ ctrl, long long slice_len,
                                                        const int *__restrict__ slice_done) {
-  constexpr int LD = NY + 1, NE = NX * NY;
+  constexpr int LD = NPL * NY + 1, NE = NX * NY, NT = NPL * NE;
   if (ctrl && ctrl->done) return;
   if (slice_done && slice_done[blockIdx.z]) return;
   __shared__ vec2<T> lds[NX * LD];
   __shared__ cx<T> tx[NX], ty[NY];
   for (int k = threadIdx.x; k < NX; k += kFftThreads) tx[k] = mk<T>((T)twx[k].x, (T)twx[k].y);
   for (int k = threadIdx.x; k < NY; k += kFftThreads) ty[k] = mk<T>((T)twy[k].x, (T)twy[k].y);
-  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len + (long long)blockIdx.x * NE;
+  vec2<T> *base =
+      F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len + (long long)blockIdx.x * NT;
+  // element e of the tile: plane e / NE, row (e % NE) / NX, column e % NX
 #pragma unroll 4
-  for (int e = threadIdx.x; e < NE; e += kFftThreads) lds[(e % NX) * LD + e / NX] = base[e];
+  for (int e = threadIdx.x; e < NT; e += kFftThreads) lds[(e % NX) * LD + e / NX] = base[e];
   __syncthreads();
   const int sign = mode == 1 ? +1 : -1;
-  fast_run<T, NX, NY, 0, 1, LD, 1>(lds, tx, sign);
-  fast_run<T, NY, NX, 0, 1, 1, LD>(lds, ty, sign);
+  fast_run<T, NX, NPL * NY, 0, 1, LD, 1>(lds, tx, sign);
+  fast_run<T, NY, NPL * NX, 0, 1, 1, LD, NX, NY>(lds, ty, sign);
 #pragma unroll 4
-  for (int e = threadIdx.x; e < NE; e += kFftThreads) base[e] = lds[(e % NX) * LD + e / NX];
+  for (int e = threadIdx.x; e < NT; e += kFftThreads) base[e] = lds[(e % NX) * LD + e / NX];
 }
 
 template <typename T, int N>
@@ -340,8 +349,16 @@ static bool try_xy(void *F, long long fstride, int nfield, const FftAxis &ax, co
                    const int *slice_done) {
   if (ax.N != N || ay.N != N) return false;
   if (sizeof(T) * 2 * N * (N + 1) > 48 * 1024) return false;
-  hipLaunchKernelGGL((k_fft_xy<T, N, N>), dim3((unsigned)nplane, nfield, nslice), dim3(kFftThreads), 0, s,
-                     (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
+  // two planes per tile in FP32 where a plane has at most 1024 points (the
+  // tile's row e / NX runs over (plane, y)); in FP64 the doubled tile
+  // halves the workgroups an LDS holds and gained nothing at 32 x 32
+  constexpr int NPL = (sizeof(T) == 4 && N * N <= 1024) ? 2 : 1;
+  if (NPL == 2 && nplane % 2 == 0)
+    hipLaunchKernelGGL((k_fft_xy<T, N, N, NPL>), dim3((unsigned)(nplane / 2), nfield, nslice), dim3(kFftThreads),
+                       0, s, (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
+  else
+    hipLaunchKernelGGL((k_fft_xy<T, N, N, 1>), dim3((unsigned)nplane, nfield, nslice), dim3(kFftThreads), 0, s,
+                       (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
   return true;
 }
 
