@@ -38,7 +38,8 @@ __global__ __launch_bounds__(kBlock) void k_lie_pack(const double2 *__restrict__
                                                     long long V) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= V) return;
-#pragma unroll 1
+  // all four directions' loads in flight at once
+#pragma unroll
   for (int mu = 0; mu < kND; mu++) {
     const su3<double> u = load_link<double>(U, mu, i, V);
     const double c0 = u.e[0].im, c4 = u.e[4].im, c8 = u.e[8].im;
