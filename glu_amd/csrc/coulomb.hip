@@ -428,8 +428,8 @@ int coulomb_alloc(glu_gfix_ctx *c) {
 
 void coulomb_free(glu_gfix_ctx *c) {
   if (!c->cs) return;
-  if (c->cs->G) hipFree(c->cs->G);
-  if (c->cs->arena) hipFree(c->cs->arena);
+  if (c->cs->G) (void)hipFree(c->cs->G);  // teardown: nothing to do on failure
+  if (c->cs->arena) (void)hipFree(c->cs->arena);
   delete c->cs;
   c->cs = nullptr;
 }
@@ -456,9 +456,10 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
   std::vector<int> done(T);
   std::vector<long long> zero(T, 0);
   for (int t = 0; t < T; t++) done[t] = active[t] ? 0 : 1;
-  hipMemcpyAsync(sc.done, done.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
-  hipMemcpyAsync(sc.iters, zero.data(), T * sizeof(long long), hipMemcpyHostToDevice, s);
-  hipStreamSynchronize(s);
+  if (hipMemcpyAsync(sc.done, done.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(sc.iters, zero.data(), T * sizeof(long long), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return gfix_fail(-2, "Coulomb slice state upload failed");
   double2 *U = (double2 *)c->U64, *G = (double2 *)cs->G, *F = (double2 *)c->F, *sn = (double2 *)c->sn,
           *O = (double2 *)c->old;
   double *part = c->part;
@@ -477,7 +478,7 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 0);
     std::swap(F, O);
   }
-  for (int round = 0;; round++) {
+  for (;;) {
     for (int b = 0; b < 8; b++) {
       hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc);
       enq_fa<double>(c, F, nullptr, sc.done);
@@ -531,18 +532,21 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     if (int rc = run_slices(c, method, active)) return rc;
     // every copy below is ordered on the context stream (a non-blocking
     // stream does not wait for the null stream, nor the null stream for it)
-    hipMemcpyAsync(th.data(), sc.theta, T * sizeof(double), hipMemcpyDeviceToHost, s);
-    hipMemcpyAsync(it.data(), sc.iters, T * sizeof(long long), hipMemcpyDeviceToHost, s);
-    hipStreamSynchronize(s);
+    if (hipMemcpyAsync(th.data(), sc.theta, T * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(it.data(), sc.iters, T * sizeof(long long), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return gfix_fail(-2, "Coulomb slice results read failed");
     // unitarity test of the converged slices (reunitarises them)
     std::vector<double> devs(T, 0.0);
     for (int t = 0; t < T; t++) sel[t] = active[t] && th[t] < acc;
-    hipMemcpyAsync(dsel, sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
+    if (hipMemcpyAsync(dsel, sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
+      return gfix_fail(-2, "Coulomb selection upload failed");
     hipLaunchKernelGGL(k_cadequate, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->part,
                        c->g, cs->nbs);
     std::vector<double> pm((size_t)cs->nbs * T);
-    hipMemcpyAsync(pm.data(), c->part, pm.size() * sizeof(double), hipMemcpyDeviceToHost, s);
-    if (hipStreamSynchronize(s) != hipSuccess) return gfix_fail(-2, "unitarity check failed");
+    if (hipMemcpyAsync(pm.data(), c->part, pm.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return gfix_fail(-2, "unitarity check failed");
     for (int t = 0; t < T; t++)
       if (sel[t])
         for (long long b = 0; b < cs->nbs; b++) devs[t] = std::max(devs[t], pm[t * cs->nbs + b]);
@@ -573,10 +577,12 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     bool anyr = false;
     for (int t = 0; t < T; t++) anyr = anyr || restart[t];
     if (anyr) {
-      hipMemcpyAsync(dsel, restart.data(), T * sizeof(int), hipMemcpyHostToDevice, s);
+      if (hipMemcpyAsync(dsel, restart.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
+        return gfix_fail(-2, "Coulomb restart upload failed");
       hipLaunchKernelGGL(k_crandom, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->g,
                          restart_seed + (uint64_t)round);
-      hipStreamSynchronize(s);  // `restart` is host memory read by the copy above
+      // `restart` is host memory read by the copy above
+      if (hipStreamSynchronize(s) != hipSuccess) return gfix_fail(-2, "Coulomb restart failed");
     }
   }
   hipLaunchKernelGGL(k_cfinal, dim3(nbl(c->g.V)), dim3(kBlock), 0, s, (double2 *)c->U64, (const double2 *)cs->G,

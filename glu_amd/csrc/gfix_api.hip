@@ -214,20 +214,22 @@ bad:
 
 extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   if (!c) return 0;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  // teardown releases everything it can; a failing release has no remedy
+  // here, so its status is dropped
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
                   c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->lie, c->F32, c->sn32, c->old32};
   for (void *b : bufs)
-    if (b) hipFree(b);
+    if (b) (void)hipFree(b);
   coulomb_free(c);
   glueprop_free(c);
   prof_harvest(c);
-  for (hipEvent_t e : c->prof_pool) hipEventDestroy(e);
+  for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
   for (hipGraphExec_t &gx : c->graph)
-    if (gx) hipGraphExecDestroy(gx);
-  if (c->run_ev) hipEventDestroy(c->run_ev);
-  if (c->hctrl) hipHostFree(c->hctrl);
-  if (c->stream) hipStreamDestroy(c->stream);
+    if (gx) (void)hipGraphExecDestroy(gx);
+  if (c->run_ev) (void)hipEventDestroy(c->run_ev);
+  if (c->hctrl) (void)hipHostFree(c->hctrl);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
 }
@@ -354,8 +356,7 @@ hipEvent_t prof_event(glu_gfix_ctx *c) {
     return e;
   }
   hipEvent_t e = nullptr;
-  hipEventCreate(&e);
-  return e;
+  return hipEventCreate(&e) == hipSuccess ? e : nullptr;
 }
 
 // collect the durations of recorded launches (call after a stream sync)
@@ -560,7 +561,7 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
     enq_iterations<T>(c);  // moves the host's buffer pointers as the batch will
     if (hipStreamEndCapture(c->stream, &gr) != hipSuccess || !gr) return gfix_fail(-2, "graph capture failed");
     const hipError_t e = hipGraphInstantiate(&c->graph[kind], gr, nullptr, nullptr, 0);
-    hipGraphDestroy(gr);
+    (void)hipGraphDestroy(gr);  // the instantiated graph no longer needs it
     if (e != hipSuccess) return gfix_fail(-2, "graph instantiation failed: %s", hipGetErrorString(e));
   } else if (c->run.method == GLU_GFIX_CG && (c->batch & 1)) {
     swap_cg_buffers<T>(c);  // a replayed batch of odd length ends on the other buffers
@@ -939,7 +940,7 @@ extern "C" int glu_gfix_stage_egauge(glu_gfix_ctx *c, const double *fields, doub
 // ---------------------------------------------------------------------------
 extern "C" int glu_gfix_profile(glu_gfix_ctx *c, int mask) {
   if (!c) return gfix_fail(-1, "null context");
-  hipStreamSynchronize(c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "profile: stream sync failed");
   prof_harvest(c);
   c->prof = mask;
   if (mask)
@@ -951,7 +952,7 @@ extern "C" int glu_gfix_profile_read(glu_gfix_ctx *c, int kernel, size_t *launch
                                      double *total_ms) {
   if (!c || kernel < 0 || kernel >= 2 * PK_COUNT || !launches || !mean_ms || !total_ms)
     return gfix_fail(-1, "glu_gfix_profile_read: bad argument");
-  hipStreamSynchronize(c->stream);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "profile read: stream sync failed");
   prof_harvest(c);
   std::vector<float> v = c->prof_ms[kernel];
   *launches = 0;

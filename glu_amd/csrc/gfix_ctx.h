@@ -102,18 +102,25 @@ struct glu_gfix_ctx {
   std::vector<float> prof_ms[2 * PK_COUNT];
 };
 
-// time one launch statement with HIP events on the context stream
-#define GFIX_PROF(c, id, stmt)                                       \
-  do {                                                              \
-    if ((c)->prof & (1 << (id))) {                                  \
-      ProfPending p_{(id), prof_event(c), prof_event(c)};           \
-      hipEventRecord(p_.a, (c)->stream);                            \
-      stmt;                                                         \
-      hipEventRecord(p_.b, (c)->stream);                            \
-      (c)->prof_pending.push_back(p_);                              \
-    } else {                                                        \
-      stmt;                                                         \
-    }                                                               \
+// time one launch statement with HIP events on the context stream; a
+// launch whose events cannot be made or recorded runs untimed
+#define GFIX_PROF(c, id, stmt)                                                          \
+  do {                                                                                 \
+    ProfPending p_{(id), nullptr, nullptr};                                            \
+    if (((c)->prof & (1 << (id))) && (p_.a = prof_event(c)) && (p_.b = prof_event(c)) && \
+        hipEventRecord(p_.a, (c)->stream) == hipSuccess) {                             \
+      stmt;                                                                            \
+      if (hipEventRecord(p_.b, (c)->stream) == hipSuccess) {                           \
+        (c)->prof_pending.push_back(p_);                                               \
+      } else {                                                                         \
+        (c)->prof_pool.push_back(p_.a);                                                \
+        (c)->prof_pool.push_back(p_.b);                                                \
+      }                                                                                \
+    } else {                                                                           \
+      if (p_.a) (c)->prof_pool.push_back(p_.a);                                        \
+      if (p_.b) (c)->prof_pool.push_back(p_.b);                                        \
+      stmt;                                                                            \
+    }                                                                                  \
   } while (0)
 hipEvent_t prof_event(glu_gfix_ctx *c);
 void prof_harvest(glu_gfix_ctx *c);

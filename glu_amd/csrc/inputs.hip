@@ -224,7 +224,7 @@ extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed) {
   hipLaunchKernelGGL(k_random_gauge, dim3(nb(c->g.V)), dim3(kBlock), 0, c->stream, (double2 *)G, c->g.V, seed);
   hipLaunchKernelGGL(k_gtransform, dim3(nb(c->g.V)), dim3(kBlock), 0, c->stream, (double2 *)c->U64,
                      (const double2 *)G, c->g);
-  hipFreeAsync(G, c->stream);
+  (void)hipFreeAsync(G, c->stream);  // a failure shows in the sync below
   if (hipStreamSynchronize(c->stream) != hipSuccess)
     return gfix_fail(-2, "random transform failed: %s", hipGetErrorString(hipGetLastError()));
   return 0;
