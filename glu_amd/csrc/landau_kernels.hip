@@ -168,9 +168,15 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
       double acc = 0.0;
 #pragma unroll 1
       for (int mu = 0; mu < kND; mu++) {
-        const su3<double> u = promote<T>(load_link<T>(U, mu, i, V));
-        if (pr == 1) v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
-        acc += retrace_abc_dag<double>(A, u, gauge_at64<T>(F, V, nb_fwd(g, i, x, mu), al));
+        // rows 0, 1 of A u first: the link is dead while the neighbour's
+        // matrix is built, which keeps the register peak lower
+        cx<double> au[6];
+        {
+          const su3<double> u = promote<T>(load_link<T>(U, mu, i, V));
+          if (pr == 1) v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
+          ab_rows01<double>(A, u, au);
+        }
+        acc += retrace_rows01_dag<double>(au, gauge_at64<T>(F, V, nb_fwd(g, i, x, mu), al));
       }
       v[pr] = acc;
     }

@@ -182,6 +182,32 @@ __device__ __forceinline__ T retrace_abc_dag(const su3<T> &a, const su3<T> &b, c
   return re + im;
 }
 
+// retrace_abc_dag in two steps, with the same arithmetic: rows 0, 1 of
+// a b, then Re tr((a b) c^dagger) with row 2 of a b from the cofactors
+template <typename T> __device__ __forceinline__ void ab_rows01(const su3<T> &a, const su3<T> &b, cx<T> (&ab)[6]) {
+#pragma unroll
+  for (int r = 0; r < 2; r++)
+#pragma unroll
+    for (int s = 0; s < 3; s++)
+      ab[r * 3 + s] = cadd(cadd(cmul(a.e[r * 3], b.e[s]), cmul(a.e[r * 3 + 1], b.e[3 + s])),
+                           cmul(a.e[r * 3 + 2], b.e[6 + s]));
+}
+template <typename T> __device__ __forceinline__ T retrace_rows01_dag(const cx<T> (&ab01)[6], const su3<T> &c) {
+  cx<T> ab[9];
+#pragma unroll
+  for (int k = 0; k < 6; k++) ab[k] = ab01[k];
+  ab[6] = cconj(csub(cmul(ab[1], ab[5]), cmul(ab[2], ab[4])));
+  ab[7] = cconj(csub(cmul(ab[2], ab[3]), cmul(ab[0], ab[5])));
+  ab[8] = cconj(csub(cmul(ab[0], ab[4]), cmul(ab[1], ab[3])));
+  T re = T(0), im = T(0);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    re += ab[k].re * c.e[k].re;
+    im += ab[k].im * c.e[k].im;
+  }
+  return re + im;
+}
+
 // product a b with columns 0,1 computed and column 2 completed: the same
 // completion as the storage format, so repeated products stay consistent
 template <typename T> __device__ __forceinline__ su3<T> mul_cols(const su3<T> &a, const su3<T> &b) {
