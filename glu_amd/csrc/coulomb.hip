@@ -98,14 +98,20 @@ __global__ __launch_bounds__(kBlock) void k_cderiv(const double2 *__restrict__ U
   coords(g, i, x);
   const su3<double> Gi = load_link<double>(G, 0, i, V);
   double sum[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-#pragma unroll
+  // one direction at a time, each rotated link projected as soon as it is
+  // formed (the same arithmetic, fewer registers held)
+#pragma unroll 1
   for (int mu = 0; mu < 3; mu++) {
     const long long it = nb_fwd(g, i, x, mu), bk = nb_bwd(g, i, x, mu);
-    const su3<double> C = gtransform<double>(Gi, load_link<double>(U, mu, i, V), load_link<double>(G, 0, it, V));
-    const su3<double> D = gtransform<double>(load_link<double>(G, 0, bk, V), load_link<double>(U, mu, bk, V), Gi);
     double a[10], b[10];
-    herm_short(C, b);
-    herm_short(D, a);
+    {
+      const su3<double> C = gtransform<double>(Gi, load_link<double>(U, mu, i, V), load_link<double>(G, 0, it, V));
+      herm_short(C, b);
+    }
+    {
+      const su3<double> D = gtransform<double>(load_link<double>(G, 0, bk, V), load_link<double>(U, mu, bk, V), Gi);
+      herm_short(D, a);
+    }
 #pragma unroll
     for (int k = 0; k < 10; k++) sum[k] += a[k] - b[k];
   }
@@ -231,13 +237,21 @@ __global__ __launch_bounds__(kBlock) void k_cline(const double2 *__restrict__ U,
     coords(g, i, x);
     const su3<double> Gi = load_link<double>(G, 0, i, V);
     const su3<double> A1 = cgauge_at(F, V, i, PC1), A2 = cgauge_at(F, V, i, PC2);
-#pragma unroll
+    // one direction at a time, rows 0, 1 of A C formed before each probe's
+    // neighbour matrix is built: the same arithmetic as retrace_abc_dag
+    // with fewer registers held
+#pragma unroll 1
     for (int mu = 0; mu < 3; mu++) {
       const long long fw = nb_fwd(g, i, x, mu);
-      const su3<double> C = gtransform<double>(Gi, load_link<double>(U, mu, i, V), load_link<double>(G, 0, fw, V));
-      v[0] += C.e[0].re + C.e[4].re + C.e[8].re;
-      v[1] += retrace_abc_dag<double>(A1, C, cgauge_at(F, V, fw, PC1));
-      v[2] += retrace_abc_dag<double>(A2, C, cgauge_at(F, V, fw, PC2));
+      cx<double> ac1[6], ac2[6];
+      {
+        const su3<double> C = gtransform<double>(Gi, load_link<double>(U, mu, i, V), load_link<double>(G, 0, fw, V));
+        v[0] += C.e[0].re + C.e[4].re + C.e[8].re;
+        ab_rows01<double>(A1, C, ac1);
+        ab_rows01<double>(A2, C, ac2);
+      }
+      v[1] += retrace_rows01_dag<double>(ac1, cgauge_at(F, V, fw, PC1));
+      v[2] += retrace_rows01_dag<double>(ac2, cgauge_at(F, V, fw, PC2));
     }
   }
   block_sum_store<3>(v, part + 3 * (t * nbs + bx));
