@@ -187,8 +187,8 @@ __global__ void k_cfin_pr(const double *__restrict__ part, long long nbs, SliceC
   }
 }
 
-// sn = in + beta sn ; old = in  (mode 0: sn = old = in)
-// sn = in + beta sn (mode 1) or sn = in (mode 0, the SD start).  GLU's
+// sn = in + beta sn (mode 1) or sn = in (mode 0, the SD start), after
+XX
 // copy old = in is not made: the host swaps its in and in_old pointers
 // after every CG iteration, so the next derivative lands in the old buffer
 // and this iteration's in becomes in_old in place.
