@@ -187,11 +187,10 @@ __global__ void k_cfin_pr(const double *__restrict__ part, long long nbs, SliceC
   }
 }
 
-// sn = in + beta sn (mode 1) or sn = in (mode 0, the SD start), after
-XX
-// copy old = in is not made: the host swaps its in and in_old pointers
-// after every CG iteration, so the next derivative lands in the old buffer
-// and this iteration's in becomes in_old in place.
+// sn = in + beta sn (mode 1, Gfix/CFACG.c:359-365) or sn = in (mode 0, the
+// SD start, :323).  GLU's copy old = in is not made: the host swaps its in
+// and in_old pointers after every CG iteration, so the next derivative
+// lands in the old buffer and this iteration's in becomes in_old in place.
 __global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ F, double2 *__restrict__ sn, Geo g,
                                                    SliceCtrl sc, int mode) {
   int t;
