@@ -534,6 +534,10 @@ std::string cut_file(const struct cut_info &ci) {
 // definition; the transform, momentum list and contraction run on the device
 extern "C" int cuts_struct(struct site *A, const struct cut_info CUTINFO) {
   const int kFailure = -1, kSuccess = 0;  // GLU_FAILURE, GLU_SUCCESS
+  if (!A) {
+    fprintf(stderr, "[CUTS] no lattice given\n");
+    return kFailure;
+  }
   if (CUTINFO.dir != GLUON_PROPS && CUTINFO.dir != EXCEPTIONAL) {
     fprintf(stderr, "[CUTS] cut mode %d is not provided by this library\n", (int)CUTINFO.dir);
     return kFailure;

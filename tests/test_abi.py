@@ -59,3 +59,24 @@ def test_oracle_is_not_linked_into_the_product():
     names = [l.split()[-1] for l in out.splitlines() if l.strip()]
     # the oracle's C symbols all start with or_
     assert not [n for n in names if n.startswith("or_")]
+
+
+class _CutInfo(C.Structure):
+    """struct cut_info, Headers/GLU_types.h:94-103"""
+    _fields_ = [("dir", C.c_int), ("type", C.c_int), ("max_mom", C.c_size_t), ("max_t", C.c_size_t),
+                ("where", C.c_char * 256), ("definition", C.c_int), ("angle", C.c_size_t),
+                ("cyl_width", C.c_double)]
+
+
+def test_cuts_struct_refusals_need_no_device():
+    """GLU's cuts_struct refuses a missing lattice, the modes this library
+    does not provide and the LOG definition with GLU_FAILURE before it
+    touches the device (GLUON_PROPS = 6, NONEXCEPTIONAL = 1, LOG_DEF = 1)."""
+    lib = C.CDLL(gl.LIB_PATH)
+    lib.cuts_struct.restype = C.c_int
+    lib.cuts_struct.argtypes = [C.c_void_p, _CutInfo]
+    site = (C.c_char * 72)()
+    assert lib.cuts_struct(None, _CutInfo(dir=6, type=1, max_mom=3)) == -1
+    assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=1, type=1, max_mom=3)) == -1
+    assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=6, type=1, max_mom=3, definition=1)) == -1
+    assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=6, type=9, max_mom=3)) == -1
