@@ -87,21 +87,40 @@ extern "C" size_t glu_gfix_footprint(const size_t dims[4], int gftype, int preci
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// The transformed axes (x, y, z, and t for Landau) go through the LDS FFT:
+// a workgroup holds P >= 1 pencils of N points (N P <= 2048) and a stage
+// runs at most 1024 butterflies, so N is limited to 2048 and must factor
+// into the radices 2, 3, 5 and 7.
+static int check_extents(const size_t dims[4], int gftype) {
+  const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
+  long double vol = 1;
+  for (int mu = 0; mu < 4; mu++) {
+    if (dims[mu] < 2 || dims[mu] > 4096) return gfix_fail(-1, "lattice extent %zu out of range", dims[mu]);
+    vol *= (long double)dims[mu];
+    if (mu >= D) continue;
+    if (dims[mu] > 2048)
+      return gfix_fail(-1, "lattice extent %zu along a transformed axis exceeds the FFT's 2048 points", dims[mu]);
+    int rad[8];
+    if (factor_radices((int)dims[mu], rad) < 0)
+      return gfix_fail(-1, "lattice extent %zu: extents must factor into 2, 3, 5 and 7", dims[mu]);
+  }
+  if (vol >= (long double)(1ll << 31))
+    return gfix_fail(-1, "lattice volume %.0Lf is beyond the supported 2^31 - 1 sites", vol);
+  return 0;
+}
+
 extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gftype, int precision) {
   if (!out || !dims) return gfix_fail(-1, "glu_gfix_create: null argument");
   *out = nullptr;
   if (gftype != GLU_GFIX_LANDAU && gftype != GLU_GFIX_COULOMB) return gfix_fail(-1, "unknown gftype %d", gftype);
   if (precision != GLU_GFIX_FP64 && precision != GLU_GFIX_MIXED)
     return gfix_fail(-1, "unknown precision %d", precision);
+  if (int rc = check_extents(dims, gftype)) return rc;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return gfix_fail(-3, "no HIP device: the gauge-fixing library has no CPU path");
   glu_gfix_ctx *c = new glu_gfix_ctx();
   for (int mu = 0; mu < 4; mu++) {
-    if (dims[mu] < 2 || dims[mu] > 4096) {
-      delete c;
-      return gfix_fail(-1, "lattice extent %zu out of range", dims[mu]);
-    }
     c->dims[mu] = dims[mu];
     c->g.L[mu] = (int)dims[mu];
   }
@@ -109,11 +128,6 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
   for (int mu = 1; mu < 4; mu++) c->g.S[mu] = c->g.S[mu - 1] * (long long)dims[mu - 1];
   c->g.V = c->g.S[3] * (long long)dims[3];
   c->g.Lcu = c->g.S[3];
-  if (c->g.V >= (1ll << 31)) {
-    const long long vol = c->g.V;
-    delete c;
-    return gfix_fail(-1, "lattice volume %lld is beyond the supported 2^31 - 1 sites", vol);
-  }
   geo_set_div(c->g);
   c->g.xcd = 0;
   if (const char *e = getenv("GLU_GFIX_XCD")) {
@@ -405,6 +419,7 @@ template <typename T> static void enq_update_deriv(glu_gfix_ctx *c) {
                                    c->stream));
   swap_cg_buffers<T>(c);
   c->deriv_ready[tix<T>()] = 1;
+  c->run.nenq++;
 }
 
 // SD start of steep_Landau_FACG, Gfix/FACG.c:249-265.  sn = in_old = in and
@@ -529,7 +544,11 @@ static int phase_ctrl_init(glu_gfix_ctx *c, int resume, double switch_theta, siz
 }
 
 template <typename T> static void enq_iterations(glu_gfix_ctx *c) {
-  for (int k = 0; k < c->batch; k++) {
+  // per-kernel timing enqueues one iteration per batch: the iteration that
+  // stops a run is then the last one enqueued, and every timed launch did
+  // its work
+  const int nb = c->prof ? 1 : c->batch;
+  for (int k = 0; k < nb; k++) {
     if (c->run.method == GLU_GFIX_CG)
       enq_cg_iter<T>(c, bufs_for<T>(c));
     else
@@ -563,8 +582,9 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
     const hipError_t e = hipGraphInstantiate(&c->graph[kind], gr, nullptr, nullptr, 0);
     (void)hipGraphDestroy(gr);  // the instantiated graph no longer needs it
     if (e != hipSuccess) return gfix_fail(-2, "graph instantiation failed: %s", hipGetErrorString(e));
-  } else if (c->run.method == GLU_GFIX_CG && (c->batch & 1)) {
-    swap_cg_buffers<T>(c);  // a replayed batch of odd length ends on the other buffers
+  } else if (c->run.method == GLU_GFIX_CG) {
+    if (c->batch & 1) swap_cg_buffers<T>(c);  // a replayed batch of odd length ends on the other buffers
+    c->run.nenq += c->batch;
   }
   return hipGraphLaunch(c->graph[kind], c->stream) == hipSuccess ? 0 : gfix_fail(-2, "graph launch failed");
 }
@@ -572,6 +592,10 @@ template <typename T> static int enq_batch_body(glu_gfix_ctx *c) {
 static int enq_batch(glu_gfix_ctx *c) {
   RunState &r = c->run;
   const bool f32 = r.phase == PH_FP32;
+  // the host mirror holds the control block as of the last batch's end (or
+  // as just written), so its update count is the device's at this point
+  r.upd0 = c->hctrl->upd;
+  r.nenq = 0;
   if (r.start == START_SD) {
     if (f32)
       enq_sd_start<float>(c, bufs_for<float>(c));
@@ -582,11 +606,49 @@ static int enq_batch(glu_gfix_ctx *c) {
   }
   r.start = START_NONE;
   if (int rc = f32 ? enq_batch_body<float>(c) : enq_batch_body<double>(c)) return rc;
+  // a launch the runtime refused (bad configuration, resources) is
+  // reported here rather than leaving the iteration silently undone
+  if (const hipError_t e = hipGetLastError(); e != hipSuccess)
+    return gfix_fail(-2, "iteration batch launch failed: %s", hipGetErrorString(e));
   if (hipMemcpyAsync(c->hctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipEventRecord(r.ev, c->stream) != hipSuccess)
     return gfix_fail(-2, "batch enqueue failed: %s", hipGetErrorString(hipGetLastError()));
   r.inflight = 1;
   return 0;
+}
+
+// After a batch: the host swapped its link and field pointers once per
+// fused update it enqueued, the device ran only those before the stop
+// (Ctrl::upd counts them).  Undo the swaps of the updates that did not run.
+static int settle_buffers(glu_gfix_ctx *c) {
+  RunState &r = c->run;
+  const long long live = c->hctrl->upd - r.upd0;
+  if (live < 0 || live > r.nenq || (live < r.nenq && !c->hctrl->done))
+    return gfix_fail(-2, "gauge update count %lld of %lld enqueued is inconsistent", live, r.nenq);
+  if ((r.nenq - live) & 1) {
+    if (r.phase == PH_FP32)
+      swap_cg_buffers<float>(c);
+    else
+      swap_cg_buffers<double>(c);
+  }
+  r.nenq = 0;
+  return 0;
+}
+
+// per-kernel timing: a line search that found theta below CG_TOL returned
+// at once; its timing is dropped (one iteration per batch while timing)
+static void drop_skipped_timings(glu_gfix_ctx *c) {
+  if (c->hctrl->did_ls) return;
+  std::vector<ProfPending> keep;
+  for (const ProfPending &p : c->prof_pending) {
+    if (p.id % PK_COUNT == PK_LSEARCH) {
+      c->prof_pool.push_back(p.a);
+      c->prof_pool.push_back(p.b);
+    } else {
+      keep.push_back(p);
+    }
+  }
+  c->prof_pending.swap(keep);
 }
 
 int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters) {
@@ -634,7 +696,11 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     if (e == hipErrorNotReady) return 0;
     if (e != hipSuccess) return gfix_fail(-2, "iteration batch failed: %s", hipGetErrorString(e));
     r.inflight = 0;
-    if (c->prof) prof_harvest(c);
+    if (int rc = settle_buffers(c)) return rc;
+    if (c->prof) {
+      drop_skipped_timings(c);
+      prof_harvest(c);
+    }
   }
   Ctrl &h = *c->hctrl;
   if (!h.done) {
@@ -954,21 +1020,13 @@ extern "C" int glu_gfix_profile_read(glu_gfix_ctx *c, int kernel, size_t *launch
     return gfix_fail(-1, "glu_gfix_profile_read: bad argument");
   if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "profile read: stream sync failed");
   prof_harvest(c);
-  std::vector<float> v = c->prof_ms[kernel];
-  *launches = 0;
-  *mean_ms = *total_ms = 0.0;
-  if (v.empty()) return 0;
-  // launches issued past convergence return at once; count only live ones
-  std::vector<float> srt = v;
-  std::sort(srt.begin(), srt.end());
-  const float med = srt[srt.size() / 2];
+  // every timed launch did its work: while timing, a batch holds one
+  // iteration (nothing is enqueued past a stop) and the timings of line
+  // searches skipped below CG_TOL are dropped when the batch ends
+  const std::vector<float> &v = c->prof_ms[kernel];
   double tot = 0.0;
-  size_t n = 0;
-  for (float ms : v)
-    if (ms >= 0.5f * med) {
-      tot += ms;
-      n++;
-    }
+  for (float ms : v) tot += ms;
+  const size_t n = v.size();
   *launches = n;
   *total_ms = tot;
   *mean_ms = n ? tot / n : 0.0;

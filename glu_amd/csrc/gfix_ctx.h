@@ -23,6 +23,9 @@ struct RunState {
   size_t max_iters = 0, max_phase = 0;
   int segments = 0;
   int inflight = 0, finished = 0;
+  // fused gauge updates enqueued in the batch in flight, and the device's
+  // count of updates that had run when it was enqueued (Ctrl::upd)
+  long long nenq = 0, upd0 = 0;
   hipEvent_t ev = nullptr;
   size_t iters = 0;
   double theta = 0.0;
@@ -54,7 +57,9 @@ struct glu_gfix_ctx {
   int deriv_ready[2] = {0, 0};
   // FP32 packed fields of the mixed schedule (in, sn, in_old)
   void *F32 = nullptr, *sn32 = nullptr, *old32 = nullptr;
-  double switch_theta = 1e-7;
+  // FP32 -> FP64 switch: 1e-11 was the fastest of the switch points
+  // measured on 32^3x64 solves
+  double switch_theta = 1e-11;
   // FP32-phase safeguards (Ctrl::sg_mode): beta capped at 3, chosen over
   // 20 gauge copies of a 32^3x64 beta = 6.0 config; the FP64 phase and
   // FP64 runs keep GLU's beta

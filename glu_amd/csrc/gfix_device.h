@@ -116,6 +116,10 @@ struct Ctrl {
   long long iters;   // iterations in the current segment
   long long max_iters;
   long long total;   // iterations over all segments (monitoring)
+  // fused gauge updates that ran (counted by the line-search finaliser
+  // that precedes each one); the host compares it with the updates it
+  // enqueued to put its buffer pointers back after a stop inside a batch
+  long long upd;
   int done;          // set after the update of the iteration that stopped
   int did_ls;        // current iteration used the line search
   int sd_start;      // currently inside the SD start

@@ -230,31 +230,19 @@ __global__ __launch_bounds__(kBlock) void k_egauge(vec2<T> *__restrict__ U, cons
 // Fout (the buffer of in_old, which this iteration no longer needs)
 // receives the packed derivative and part its theta partials; the host
 // then swaps the link buffers and the field buffers Fin <-> Fout.  Past
-// convergence the links are copied unchanged and Fin and Fout exchange
-// their contents, so the swaps the host made at enqueue time hold
-// whichever iteration stopped, with every buffer in its role.
+// convergence the launch does nothing: the control block counts the
+// updates that ran (Ctrl::upd), and the host undoes the swaps of those
+// that did not once the batch has ended.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_egauge_deriv(const vec2<T> *__restrict__ Uin, vec2<T> *__restrict__ Uout,
                                                         const vec2<T> *__restrict__ D, double2 *__restrict__ G,
                                                         vec2<T> *__restrict__ Fin, vec2<T> *__restrict__ Fout,
                                                         double *__restrict__ part, Geo g, const Ctrl *__restrict__ ctrl) {
+  if (ctrl->done) return;
   const long long V = g.V;
   const unsigned lb = logical_block(g);
   const long long i = (long long)lb * blockDim.x + threadIdx.x;
-  if (ctrl->done) {
-    if (i < V) {
-#pragma unroll 4
-      for (int k = 0; k < kND * 6; k++) Uout[(long long)k * V + i] = Uin[(long long)k * V + i];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        const vec2<T> a = Fin[(long long)k * V + i], b = Fout[(long long)k * V + i];
-        Fin[(long long)k * V + i] = b;
-        Fout[(long long)k * V + i] = a;
-      }
-    }
-    return;
-  }
   double loc[1] = {0.0};
   if (i < V) {
     const double alpha = ctrl->alpha;
@@ -379,6 +367,7 @@ __global__ void k_fin_ls(const double *__restrict__ part, long long nb, Ctrl *ct
   double s[3] = {0.0, 0.0, 0.0};
   if (do_ls) final_sum<3>(part, nb, s);
   if (threadIdx.x == 0) {
+    ctrl->upd++;  // the gauge update enqueued after this launch will run
     if (do_ls) {
       double val[3];
       for (int k = 0; k < 3; k++) {

@@ -237,7 +237,13 @@ int glu_gfix_write_nersc(glu_gfix_ctx *ctx, const char *path, int two_rows, int 
                          const char *label, size_t sequence);
 
 /* Mixed precision: run FP32 until theta < switch_theta, then FP64 (default
-   1e-7, or the GLU_GFIX_SWITCH_THETA environment variable at creation).
+   1e-11, the fastest switch point measured on 32^3x64 solves, or the
+   GLU_GFIX_SWITCH_THETA environment variable at creation).  A mixed
+   result satisfies the same theta and plaquette bars as an FP64 one but
+   usually lies on a different Gribov copy (a neighbouring local maximum
+   of the gauge functional, 1e-6..1e-3 away in relative functional): the
+   FP32 trajectory leaves the FP64 one early.  Use GLU_GFIX_FP64 where the
+   copy GLU's own FP64 iteration reaches is required.
    The FP32 phase caps the Polak-Ribiere beta at 3 (GLU_GFIX_BETA_CAP;
    GLU_GFIX_SAFEGUARD=0 turns it off): near its rounding floor the FP32
    gradient makes PR+ directions run uphill for hundreds of iterations,
@@ -282,8 +288,10 @@ int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
    the next derivative (FACG), 7 Lie-field packing and 8 momentum
    projection of the gluon propagators; +9 for the FP32 variants.
    `mask` selects the classes timed (bit k = class k; -1 all, 0 off).
-   profile_read reports the launches that did work (launches issued past
-   convergence return at once and are left out), their mean and total. */
+   While timing is on, a Landau run enqueues one iteration per batch, so
+   no timed launch falls past the stop, and line searches skipped below
+   CG_TOL are not counted: profile_read reports launches that did their
+   work, their mean and total. */
 int glu_gfix_profile(glu_gfix_ctx *ctx, int mask);
 int glu_gfix_profile_read(glu_gfix_ctx *ctx, int kernel, size_t *launches,
                           double *mean_ms, double *total_ms);

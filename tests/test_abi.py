@@ -54,6 +54,29 @@ def test_argument_errors_are_reported():
     assert gl.footprint((32, 32, 32, 64)) > 32**3 * 64 * 576
 
 
+@pytest.mark.parametrize("dims,gftype,word", [
+    ((2560, 4, 4, 4), 0, "2048"),      # beyond the LDS FFT's pencil length
+    ((4, 4, 4, 4096), 0, "2048"),      # t is transformed in Landau ...
+    ((4, 4, 4, 5000), 1, "range"),     # ... not in Coulomb, but still capped
+    ((11, 8, 8, 8), 0, "factor"),      # no radix 11 kernel
+    ((8, 8, 8, 13), 1, None),          # Coulomb does not transform t: accepted
+    ((2048, 2048, 512, 2), 0, "volume"),
+    ((1, 8, 8, 8), 0, "range"),
+])
+def test_extent_refusals_need_no_device(dims, gftype, word):
+    """Extents the kernels cannot handle are refused before any device
+    work, with a message naming the limit."""
+    h = C.c_void_p()
+    rc = gl.lib().glu_gfix_create(C.byref(h), (C.c_size_t * 4)(*dims), gftype, 0)
+    if word is None:
+        if rc == 0:
+            gl.lib().glu_gfix_destroy(h)
+        assert rc in (0, -3)  # accepted; -3 only for the missing device
+        return
+    assert rc == -1 and not h.value
+    assert word in gl.lib().glu_gfix_last_error().decode()
+
+
 def test_oracle_is_not_linked_into_the_product():
     out = os.popen(f"nm -D --undefined-only {gl.LIB_PATH}").read()
     names = [l.split()[-1] for l in out.splitlines() if l.strip()]

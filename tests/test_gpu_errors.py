@@ -16,7 +16,7 @@ def create(dims, gftype=0, precision=0):
     return rc, h
 
 
-@pytest.mark.parametrize("dims", [(1, 8, 8, 8), (8, 8, 8, 5000), (11, 8, 8, 8), (8, 8, 8, 13)])
+@pytest.mark.parametrize("dims", [(1, 8, 8, 8), (8, 8, 8, 5000), (11, 8, 8, 8), (8, 8, 8, 13), (2560, 4, 4, 4)])
 def test_bad_extents_are_refused(dims):
     rc, h = create(dims)
     assert rc < 0 and not h.value
