@@ -174,6 +174,27 @@ int or_gluon_exceptional(const or_lat *L, const double *U, int type,
 int or_write_glueprop(const char *path, size_t n, const int *mom,
                       const double *trans, const double *longi);
 
+/* ---------------- planted mistakes (tests of the pins) ------------------- */
+/* The loop pins of tests/test_oracle_loops.py are shown to catch plausible
+   slips by planting each one here, on the CPU, and checking that a pin
+   fails.  0 = the faithful restatement (the default). */
+enum {
+  OR_PLANT_NONE = 0,
+  OR_PLANT_FR_ALWAYS = 1,    /* Fletcher-Reeves beta also above CG_TOL     */
+  OR_PLANT_MAX_FACTOR = 2,   /* psq with 4 (D - 1) / vol instead of 4 D    */
+  OR_PLANT_THETA_AFTER = 3,  /* theta_k recorded after update k           */
+  OR_PLANT_EGAUGE_SIGN = 4,  /* gauge update along -alpha                 */
+  OR_PLANT_NO_CLAMP = 5,     /* Polak-Ribiere beta not clipped at 0       */
+  OR_PLANT_NO_BETA = 6,      /* beta = 0: steepest descent directions     */
+  OR_PLANT_NO_STALE = 7,     /* continuation without GLU's stale red[] slots */
+  OR_PLANT_OLD_KEPT = 8      /* in_old not refreshed after an iteration   */
+};
+void or_set_plant(int code);
+
+/* Coulomb: record the trajectory of one slice (theta_k, step, beta) of the
+   next or_coulomb calls into *tr (NULL: off) */
+void or_coulomb_trace(or_trace *tr, size_t slice);
+
 /* ---------------- input generation (not on the hot path) ---------------- */
 /* quenched Wilson heatbath + overrelaxation, Cabibbo-Marinari with
    Kennedy-Pendleton SU(2) heatbath; own seeded generator */

@@ -387,6 +387,9 @@ double or_fa_deriv(const or_lat *L, const double *U, double *F)
   return th;
 }
 
+int or_plant = OR_PLANT_NONE;
+void or_set_plant(int code) { or_plant = code; }
+
 /* theta_test_lin = 2 sum tr(Delta^2) / (NC V): Gfix/gftests.c:215-250 */
 double or_theta_lin(const or_lat *L, const double *U, int max_dir)
 {

@@ -123,11 +123,30 @@ static void sum_pr(cl_ctx *C, const double *Fin, const double *Fold)
   }
 }
 
+/* trajectory of one slice (tests): or_coulomb_trace */
+static or_trace *cl_trace = NULL;
+static size_t cl_trace_slice = 0;
+void or_coulomb_trace(or_trace *tr, size_t slice)
+{
+  cl_trace = tr;
+  cl_trace_slice = slice;
+}
+static void cl_rec(size_t t, double theta, double alpha, double beta)
+{
+  or_trace *tr = cl_trace;
+  if (!tr || t != cl_trace_slice || tr->n >= tr->cap) return;
+  if (tr->theta) tr->theta[tr->n] = theta;
+  if (tr->alpha) tr->alpha[tr->n] = alpha;
+  if (tr->beta) tr->beta[tr->n] = beta;
+  tr->n++;
+}
+
 /* gauge(x) <- P(1 + alpha X(F(x))) gauge(x): Gfix/line_search.c:101-115 */
 static void exponentiate_gauge_cg(cl_ctx *C, double *g, const double *F,
                                   double alpha)
 {
   const size_t n = C->L->Lcu;
+  if (or_plant == OR_PLANT_EGAUGE_SIGN) alpha = -alpha;
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < n; i++) {
     double tmp[18], old[18];
@@ -223,7 +242,7 @@ static size_t steep_step_facg(cl_ctx *C, const double *U, double *g, double *tr,
   sum_der2(C, C->in);
   double inold = redsum(C, OR_SLOT_PR0);
   *tr = inold * norm;
-  line_search_coulomb(C, U, g, C->in, t);
+  cl_rec(t, *tr, line_search_coulomb(C, U, g, C->in, t), 0.0);
   memcpy(C->sn, C->in, n4 * 2 * sizeof(double));
   memcpy(C->in_old, C->in, n4 * 2 * sizeof(double));
   size_t loc = 0;
@@ -235,18 +254,21 @@ static size_t steep_step_facg(cl_ctx *C, const double *U, double *g, double *tr,
     const double insum = redsum(C, OR_SLOT_PR0), sc = redsum(C, OR_SLOT_PR1);
     *tr = insum * norm;
     double beta = sc / inold;
-    if (beta < 0.0) beta = 0.0;
-    if (*tr < OR_CG_TOL) beta = insum / inold;
+    if (beta < 0.0 && or_plant != OR_PLANT_NO_CLAMP) beta = 0.0;
+    if (*tr < OR_CG_TOL || or_plant == OR_PLANT_FR_ALWAYS) beta = insum / inold;
+    if (or_plant == OR_PLANT_NO_BETA) beta = 0.0;
     inold = insum;
     cplx *in = (cplx *)C->in, *sn = (cplx *)C->sn, *old = (cplx *)C->in_old;
     for (size_t j = 0; j < n4; j++) {
       sn[j] = in[j] + beta * sn[j];
-      old[j] = in[j];
+      if (or_plant != OR_PLANT_OLD_KEPT) old[j] = in[j];
     }
+    double step = C->gf_alpha;
     if (*tr > OR_CG_TOL)
-      line_search_coulomb(C, U, g, C->sn, t);
+      step = line_search_coulomb(C, U, g, C->sn, t);
     else
       exponentiate_gauge_cg(C, g, C->sn, C->gf_alpha);
+    cl_rec(t, *tr, step, beta);
     loc++;
     if (!(*tr > acc && loc < max_iters)) break;
   }
@@ -266,6 +288,7 @@ static size_t steep_step_fasd(cl_ctx *C, const double *U, double *g, double *tr,
     exponentiate_gauge_cg(C, g, C->in, C->gf_alpha);
     sum_der2(C, C->in);
     *tr = redsum(C, OR_SLOT_PR0) * norm;
+    cl_rec(t, *tr, C->gf_alpha, 0.0);
     loc++;
   } while (*tr > acc && loc < max_iters);
   return loc;

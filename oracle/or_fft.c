@@ -115,7 +115,7 @@ void or_dft_naive(double *data, const size_t *dims, int nd, int sign)
 void or_psq_table(const or_lat *L, int D, double *psq)
 {
   const size_t vol = D == OR_ND ? L->V : L->Lcu;
-  const double mx = 4.0 * D / (double)vol;
+  const double mx = 4.0 * (or_plant == OR_PLANT_MAX_FACTOR ? D - 1 : D) / (double)vol;
   for (size_t i = 0; i < vol; i++) psq[i] = mx / or_psq(L, i, D);
 }
 

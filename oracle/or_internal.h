@@ -59,4 +59,7 @@ static inline size_t or_blk_hi(size_t n, int nth, int t)
 double or_fa_deriv_red(const or_lat *L, const double *U, double *F,
                        double *red);
 
+/* the planted mistake in force (OR_PLANT_*; 0 in every faithful run) */
+extern int or_plant;
+
 #endif

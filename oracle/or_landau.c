@@ -16,6 +16,7 @@
 void or_egauge_landau(const or_lat *L, double *U, const double *F, double alpha)
 {
   const size_t V = L->V;
+  if (or_plant == OR_PLANT_EGAUGE_SIGN) alpha = -alpha;
 #pragma omp parallel for schedule(static)
   for (size_t i = 0; i < V; i++) {
     double A[18], B[18];
@@ -164,10 +165,11 @@ static size_t steep_landau_facg(const or_lat *L, double *U, facg_tmp *T,
   size_t loc_iters = 0;
   /* SD start (FACG.c:249-254): the red slots are NOT cleared here, so a
      continuation run sees the previous segment's values */
+  if (or_plant == OR_PLANT_NO_STALE) red_zero(L, T->red);
   *tr = or_fa_deriv_red(L, U, T->in, T->red);
   or_fourier_accelerate(L, T->in, T->psq, OR_ND, 4);
   const double a0 = or_line_search_landau(L, U, T->in, T->red);
-  rec(trc, *tr, a0, 0.0, L, U);
+  rec(trc, or_plant == OR_PLANT_THETA_AFTER ? 0.5 * or_theta_lin(L, U, OR_ND) : *tr, a0, 0.0, L, U);
   memcpy(T->sn, T->in, n4 * 2 * sizeof(double));
   memcpy(T->in_old, T->in, n4 * 2 * sizeof(double));
   sum_der3(L, T->in, T->red);
@@ -183,14 +185,16 @@ static size_t steep_landau_facg(const or_lat *L, double *U, facg_tmp *T,
     const double insum = red_sum(L, T->red, OR_SLOT_PR0);
     const double sum_conj = red_sum(L, T->red, OR_SLOT_PR1);
     double beta = sum_conj / inold;
-    if (beta < 0.0) beta = 0.0; /* PRfmax(0, b): Gfix/CG.c:38-42 */
-    if (*tr < OR_CG_TOL) beta = insum / inold; /* Fletcher-Reeves */
+    if (beta < 0.0 && or_plant != OR_PLANT_NO_CLAMP) beta = 0.0; /* PRfmax(0, b): Gfix/CG.c:38-42 */
+    if (*tr < OR_CG_TOL || or_plant == OR_PLANT_FR_ALWAYS) beta = insum / inold; /* Fletcher-Reeves */
+    if (or_plant == OR_PLANT_NO_BETA) beta = 0.0;
     inold = insum;
     cplx *in = (cplx *)T->in, *sn = (cplx *)T->sn, *old = (cplx *)T->in_old;
+    const int keep_old = or_plant == OR_PLANT_OLD_KEPT;
 #pragma omp parallel for schedule(static)
     for (size_t j = 0; j < n4; j++) {
       sn[j] = in[j] + beta * sn[j];
-      old[j] = in[j];
+      if (!keep_old) old[j] = in[j];
     }
     double step;
     if (*tr > OR_CG_TOL) {
@@ -200,7 +204,7 @@ static size_t steep_landau_facg(const or_lat *L, double *U, facg_tmp *T,
       step = gf_alpha;
     }
     loc_iters++;
-    rec(trc, *tr, step, beta, L, U);
+    rec(trc, or_plant == OR_PLANT_THETA_AFTER ? 0.5 * or_theta_lin(L, U, OR_ND) : *tr, step, beta, L, U);
     if (!(*tr > acc && loc_iters < max_iters)) break;
   }
   return loc_iters;
@@ -262,7 +266,8 @@ size_t or_fasd(const or_lat *L, double *U, double gf_alpha, double acc,
       or_fourier_accelerate(L, in, psq, OR_ND, 4);
       or_egauge_landau(L, U, in, gf_alpha);
       loc++;
-      rec(trc, *theta, gf_alpha, 0.0, L, U);
+      rec(trc, or_plant == OR_PLANT_THETA_AFTER ? 0.5 * or_theta_lin(L, U, OR_ND) : *theta, gf_alpha, 0.0,
+          L, U);
     } while (*theta > acc && loc < max_iters);
     if (loc >= max_iters) {
       if (*theta < 1e3 * acc) {

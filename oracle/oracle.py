@@ -105,6 +105,8 @@ def lib():
                                                C.POINTER(SZ)]),
             "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.c_int]),
+            "or_set_plant": (None, [C.c_int]),
+            "or_coulomb_trace": (None, [C.POINTER(Trace), SZ]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -225,13 +227,47 @@ class Lattice:
         return out
 
     def coulomb(self, U, gf_alpha=0.09, acc=1e-14, max_iters=650, facg=True,
-                seed=1):
+                seed=1, trace_slice=None, trace=0):
+        """Coulomb_FA; with trace_slice set, the trajectory (theta_k, step,
+        beta_k) of that slice is returned as trace_theta/alpha/beta."""
         T = self.dims[3]
         its = (SZ * T)()
         th = np.zeros(T)
-        tot = lib().or_coulomb(self.ref, _dp(U), gf_alpha, acc, max_iters,
-                               int(facg), seed, its, _dp(th))
-        return {"iters": int(tot), "slice_iters": list(its), "slice_theta": th}
+        tr, bufs = None, {}
+        if trace_slice is not None:
+            tr = Trace()
+            tr.cap, tr.n = trace, 0
+            for k in ("theta", "alpha", "beta"):
+                bufs[k] = np.zeros(trace)
+                setattr(tr, k, _dp(bufs[k]))
+            lib().or_coulomb_trace(C.byref(tr), trace_slice)
+        try:
+            tot = lib().or_coulomb(self.ref, _dp(U), gf_alpha, acc, max_iters,
+                                   int(facg), seed, its, _dp(th))
+        finally:
+            if tr is not None:
+                lib().or_coulomb_trace(None, 0)
+        out = {"iters": int(tot), "slice_iters": list(its), "slice_theta": th}
+        if tr is not None:
+            for k, v in bufs.items():
+                out["trace_" + k] = v[: tr.n].copy()
+        return out
+
+
+class planted:
+    """Context manager: run the oracle with one planted mistake
+    (OR_PLANT_* of glu_oracle.h), to show that a pin catches it."""
+    FR_ALWAYS, MAX_FACTOR, THETA_AFTER, EGAUGE_SIGN, NO_CLAMP, NO_BETA, NO_STALE, OLD_KEPT = range(1, 9)
+
+    def __init__(self, code: int):
+        self.code = code
+
+    def __enter__(self):
+        lib().or_set_plant(self.code)
+        return self
+
+    def __exit__(self, *a):
+        lib().or_set_plant(0)
 
 
 def fft(data: np.ndarray, dims, nd: int, sign: int) -> np.ndarray:

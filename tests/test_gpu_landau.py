@@ -19,7 +19,7 @@ import pytest
 
 from oracle import oracle as O
 
-from gpu_common import heatbath_config, rel
+from gpu_common import check_trajectory, heatbath_config, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -126,15 +126,8 @@ def test_facg_trajectory_and_final(dims):
         raw = g.measure()
         g.reunit()  # as GLU's Landau() does before its report (Gfix/Landau.c:48)
         m = g.measure()
-    n = 21  # SD start + 20 iterations
-    for col, key in enumerate(("trace_theta", "trace_alpha", "trace_beta")):
-        o1, o8, gg = r1[key][:n], r8[key][:n], tr[:n, col]
-        # rounding that entered the trajectory stays in it, so the oracle's
-        # own sensitivity at iteration k is the largest relative spread up to k
-        spread = np.maximum.accumulate(rel(o8, o1)) * np.abs(o1)
-        tol = np.maximum(1e-12 * np.abs(o1), 20 * spread)
-        bad = np.abs(gg - o1) > tol
-        assert not bad.any(), (key, np.nonzero(bad)[0], gg[bad], o1[bad], spread[bad])
+    # SD start + 20 iterations
+    check_trajectory(f"facg {dims}", tr, r1, r8, n=21)
     assert th < 1e-14
     assert abs(it - r1["iters"]) <= max(2, r1["iters"] // 20)
     assert rel(m["functional"], L.functional(W1)) < 1e-10
