@@ -35,7 +35,8 @@ EXPORTS = [
     "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
-    "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional",
+    "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional", "glu_gfix_rng_seed", "glu_gfix_glu_random_transform",
+    "glu_gfix_rng_draws",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
@@ -100,6 +101,9 @@ def lib():
             "glu_gfix_server_wait": (C.c_int, [P, C.c_int, DP, C.POINTER(SZ), DP]),
             "glu_gfix_server_stats": (C.c_int, [P, C.POINTER(SZ), DP, DP, C.POINTER(SZ)]),
             "glu_gfix_server_destroy": (C.c_int, [P]),
+            "glu_gfix_rng_seed": (C.c_int, [P, C.c_uint32, C.c_int]),
+            "glu_gfix_glu_random_transform": (C.c_int, [P]),
+            "glu_gfix_rng_draws": (C.c_int, [C.c_uint32, C.c_int, C.c_int, SZ, DP]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -271,6 +275,14 @@ class Gfix:
     def random_transform(self, seed=1):
         _check(lib().glu_gfix_random_transform(self._h, seed), "random_transform")
 
+    def rng_seed(self, seed: int, nthreads: int):
+        """GLU's initialise_par_rng on the context's device streams."""
+        _check(lib().glu_gfix_rng_seed(self._h, seed, nthreads), "rng_seed")
+
+    def glu_random_transform(self):
+        """GLU's random_transform drawn from the context's streams."""
+        _check(lib().glu_gfix_glu_random_transform(self._h), "glu_random_transform")
+
     def profile(self, mask=-1):
         """Time kernel classes with HIP events (bit k = KERNELS[k], bit
         k + len(KERNELS) its FP32 variant); -1 all, 0 off."""
@@ -312,6 +324,13 @@ class Gfix:
     def stage_egauge(self, F, alpha):
         F = np.ascontiguousarray(F, np.complex128)
         _check(lib().glu_gfix_stage_egauge(self._h, _dp(F), alpha), "stage_egauge")
+
+
+def rng_draws(seed: int, nthreads: int, thread: int, n: int) -> np.ndarray:
+    """par_rng_dbl(thread) of GLU's streams seeded (seed, nthreads)."""
+    out = np.zeros(n)
+    _check(lib().glu_gfix_rng_draws(seed, nthreads, thread, n, _dp(out)), "rng_draws")
+    return out
 
 
 def footprint(dims, gftype=LANDAU, precision=FP64) -> int:

@@ -355,33 +355,6 @@ __global__ __launch_bounds__(kBlock) void k_cadequate(double2 *__restrict__ G, c
   }
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t z) {
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-// random SU(3) restart of the selected slices (Gfix/CFACG.c:497-522)
-__global__ __launch_bounds__(kBlock) void k_crandom(double2 *__restrict__ G, const int *__restrict__ sel, Geo g,
-                                                   uint64_t seed) {
-  const int t = blockIdx.y;
-  if (!sel[t]) return;
-  const long long xl = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (xl >= g.Lcu) return;
-  const long long i = site_of(g, t, xl);
-  uint64_t s = mix64(seed ^ mix64((uint64_t)i));
-  su3<double> m;
-  for (int k = 0; k < 9; k++) {
-    s += 0x9E3779B97F4A7C15ULL;
-    const double a = ((double)(mix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-    s += 0x9E3779B97F4A7C15ULL;
-    const double b = ((double)(mix64(s) >> 11) + 1.0) * (1.0 / 9007199254740992.0);
-    const double r = sqrt(-2.0 * log(a));
-    m.e[k] = mk<double>(r * cos(6.283185307179586 * b), r * sin(6.283185307179586 * b));
-  }
-  gram_reunit<double>(m);
-  store_link<double>(G, 0, i, g.V, m);
-}
 
 __global__ __launch_bounds__(kBlock) void k_cidentity(double2 *__restrict__ G, long long V) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -537,12 +510,21 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
   std::vector<double> th(T);
   std::vector<long long> it(T);
   int *dsel = sc.did_ls;  // reused as a device selection mask between rounds
-  uint64_t restart_seed = 0x9a7b3c1dULL;
-  for (int round = 0; round < 4096; round++) {
-    bool any = false;
-    for (int t = 0; t < T; t++) any = any || active[t];
-    if (!any) break;
-    if (int rc = run_slices(c, method, active)) return rc;
+  // slices waiting for their random restart: GLU fixes the slices one
+  // after another, so the restarts of slice t draw from the streams after
+  // every draw of the slices before it; a slice that needs one waits until
+  // all slices before it have finished (Gfix/CFACG.c:505-521, 593-614)
+  std::vector<int> pending(T, 0);
+  int rc_all = 0;
+  for (int round = 0; round < 4096 && !rc_all; round++) {
+    bool any = false, anyp = false;
+    for (int t = 0; t < T; t++) {
+      any = any || active[t];
+      anyp = anyp || pending[t];
+    }
+    if (!any && !anyp) break;
+    if (any)
+      if (int rc = run_slices(c, method, active)) return rc;
     // every copy below is ordered on the context stream (a non-blocking
     // stream does not wait for the null stream, nor the null stream for it)
     if (hipMemcpyAsync(th.data(), sc.theta, T * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
@@ -552,6 +534,7 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     // unitarity test of the converged slices (reunitarises them)
     std::vector<double> devs(T, 0.0);
     for (int t = 0; t < T; t++) sel[t] = active[t] && th[t] < acc;
+    if (!any) std::fill(sel.begin(), sel.end(), 0);
     if (hipMemcpyAsync(dsel, sel.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
       return gfix_fail(-2, "Coulomb selection upload failed");
     hipLaunchKernelGGL(k_cadequate, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->part,
@@ -563,7 +546,6 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
     for (int t = 0; t < T; t++)
       if (sel[t])
         for (long long b = 0; b < cs->nbs; b++) devs[t] = std::max(devs[t], pm[t * cs->nbs + b]);
-    std::vector<int> restart(T, 0);
     for (int t = 0; t < T; t++) {
       if (!active[t]) continue;
       const size_t loc = (size_t)it[t];
@@ -578,7 +560,8 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
       }
       if (control[t] < 8 && trans) {
         if (c->verbose) fprintf(stdout, "[CG] random gauge transform %d :: %e (%zu)\n", control[t], th[t], loc);
-        restart[t] = 1;
+        pending[t] = 1;
+        active[t] = 0;
         tot[t] += loc;
         control[t]++;
         continue;
@@ -587,17 +570,23 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
       active[t] = 0;
       cs->slice_theta[t] = th[t];
     }
-    bool anyr = false;
-    for (int t = 0; t < T; t++) anyr = anyr || restart[t];
-    if (anyr) {
-      if (hipMemcpyAsync(dsel, restart.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess)
-        return gfix_fail(-2, "Coulomb restart upload failed");
-      hipLaunchKernelGGL(k_crandom, dim3((unsigned)cs->nbs, T), dim3(kBlock), 0, s, (double2 *)cs->G, dsel, c->g,
-                         restart_seed + (uint64_t)round);
-      // `restart` is host memory read by the copy above
-      if (hipStreamSynchronize(s) != hipSuccess) return gfix_fail(-2, "Coulomb restart failed");
+    // draw, in slice order, for the waiting slices whose predecessors have
+    // all finished; Sunitary_gen sets the slice transform itself
+    std::vector<long long> first;
+    bool blocked = false;
+    for (int t = 0; t < T && !blocked; t++) {
+      if (active[t]) blocked = true;
+      if (!pending[t]) continue;
+      if (blocked) break;
+      first.push_back((long long)t * c->g.Lcu);
+      pending[t] = 0;
+      active[t] = 1;
+      blocked = true;  // its own new attempt must end before later slices draw
     }
+    if (!first.empty()) rc_all = mwc_sunitary(c, cs->G, c->g.V, c->g.Lcu, first, true);
   }
+  c->mwc_live = 0;  // Coulomb_FA frees GLU's streams on return (CFACG.c:630)
+  if (rc_all) return rc_all;
   hipLaunchKernelGGL(k_cfinal, dim3(nbl(c->g.V)), dim3(kBlock), 0, s, (double2 *)c->U64, (const double2 *)cs->G,
                      c->g);
   if (hipStreamSynchronize(s) != hipSuccess)

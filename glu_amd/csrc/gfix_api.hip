@@ -232,7 +232,7 @@ extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
   // here, so its status is dropped
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   void *bufs[] = {c->U64, c->U32, c->U64b, c->U32b, c->G64, c->Usnap, c->Uorig, c->Ubest, c->F, c->sn, c->old, c->part, c->ctrl, c->trace,
-                  c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->lie, c->F32, c->sn32, c->old32};
+                  c->cosv, c->tw[0], c->tw[1], c->tw[2], c->tw[3], c->stage, c->lie, c->F32, c->sn32, c->old32, c->mwc};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   coulomb_free(c);
@@ -778,9 +778,12 @@ int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_it
 // (Headers/GLU_definitions.h:400-402).  Failed attempts add max_iters to
 // the count, the successful one its own iterations, so a restarted run
 // reports 123456789 + ... as GLU does.  GLU re-reads the config file and
-// draws the transform from its thread-dependent MWC streams; here the input
-// comes from device memory and restart k uses the device transform seeded
-// restart_seed + k.  *failures: restarts taken (7 = gave up).
+// applies random_transform (Geometry/random_config.c:108-120) drawn from
+// its MWC_4096 streams; here the input comes from a device copy and the
+// transform from the context's copy of those streams: seeded with
+// restart_seed at the first restart (GLU's fresh process with SEED =
+// restart_seed), or continuing the context's streams when restart_seed is
+// 0.  *failures: restarts taken (7 = gave up).
 int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters) {
   if (loc != 123456789) {
     *iters = first + loc;
@@ -808,7 +811,10 @@ extern "C" int glu_gfix_landau(glu_gfix_ctx *c, int method, double gf_tune, doub
     for (fail = 1; fail < kGluFailures; fail++) {
       if (hipMemcpyAsync(c->U64, c->Uorig, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
         return gfix_fail(-2, "input reload failed");
-      if (int rc = glu_gfix_random_transform(c, restart_seed + (uint64_t)fail)) return rc;
+      if (fail == 1 && restart_seed) {
+        if (int rc = mwc_seed(c, (uint32_t)restart_seed, c->rng_threads)) return rc;
+      }
+      if (int rc = mwc_random_transform(c)) return rc;
       size_t loc = 0;
       if (int rc = landau_fix(c, method, gf_tune, accuracy, max_iters, theta, &loc)) return rc;
       if (landau_restart_policy(total, loc, max_iters, &total)) break;
@@ -826,10 +832,11 @@ extern "C" int glu_gfix_landau(glu_gfix_ctx *c, int method, double gf_tune, doub
 // under a random gauge transform, fixed to 1e-6 (at most 1000 iterations);
 // the copy with the smallest functional among those that neither failed
 // nor used all 1000 iterations is fixed to `accuracy`.  The count is the
-// last copy's iterations plus the final run's, as in GLU.  Copy k uses the
-// device transform seeded seed + k (GLU draws from its MWC streams).  If
-// no copy qualifies GLU continues from an unset buffer; here the final run
-// starts from the input.
+// last copy's iterations plus the final run's, as in GLU.  Each copy's
+// random_transform draws from the context's MWC_4096 streams, seeded with
+// `seed` at the start (or continuing when seed is 0), as GLU's copies draw
+// from its process streams.  If no copy qualifies GLU continues from an
+// unset buffer; here the final run starts from the input.
 extern "C" int glu_gfix_luxury(glu_gfix_ctx *c, int method, double gf_tune, double accuracy, size_t max_iters,
                                int copies, uint64_t seed, double *theta, size_t *iters, int *best_copy) {
   if (!c || !theta || !iters) return gfix_fail(-1, "glu_gfix_luxury: null argument");
@@ -845,10 +852,11 @@ extern "C" int glu_gfix_luxury(glu_gfix_ctx *c, int method, double gf_tune, doub
   double best = 1.0;  // GLU's maxlink under BEST_COPY
   int chosen = -1;
   size_t it = 0;
+  if (int rc = seed ? mwc_seed(c, (uint32_t)seed, c->rng_threads) : mwc_ensure(c)) return rc;
   for (int k = 0; k < copies; k++) {
     if (hipMemcpyAsync(c->U64, c->Uorig, bytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
       return gfix_fail(-2, "input reload failed");
-    if (int rc = glu_gfix_random_transform(c, seed + (uint64_t)k)) return rc;
+    if (int rc = mwc_random_transform(c)) return rc;
     double th = 0.0, f = 0.0;
     if (int rc = landau_fix(c, method, gf_tune, 1e-6, 1000, &th, &it)) return rc;
     if (int rc = glu_gfix_measure(c, nullptr, nullptr, nullptr, &f, nullptr)) return rc;

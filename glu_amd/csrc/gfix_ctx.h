@@ -99,6 +99,14 @@ struct glu_gfix_ctx {
   void *glue = nullptr;   // device copy of the last momentum list measured
   size_t last_fp32_iters = 0, last_fp64_iters = 0;
   CoulombState *cs = nullptr;
+  // GLU's MWC_4096 streams on the device (x[mwc_nth], carry[mwc_nth]):
+  // restarts, luxury copies and glu_gfix_glu_random_transform draw from
+  // them; live once seeded (GLU's RNG_inited).  rng_seed / rng_threads
+  // seed them when a run needs them unseeded (GLU's Latt.Seed, Nthreads).
+  void *mwc = nullptr;
+  int mwc_nth = 0, mwc_live = 0;
+  uint32_t rng_seed = 1;
+  int rng_threads = 256;
   // profiling: id = kernel class + PK_COUNT for the FP32 variants; prof is
   // a bit mask of the ids being timed
   int prof = 0;
@@ -151,3 +159,21 @@ extern "C" int glu_gfix_reunit(glu_gfix_ctx *c);
 constexpr long long kChunk = 1 << 17;
 int ensure_stage(glu_gfix_ctx *c);
 void glueprop_free(glu_gfix_ctx *c);
+
+// GLU's MWC_4096 streams (mwc.hip)
+struct MwcHost {
+  std::vector<uint32_t> x, carry;
+  bool live = false;
+};
+extern MwcHost g_mwc;  // the process stream of the drop-in entry points
+void mwc_seed_states(uint32_t seed, int nth, uint32_t *x, uint32_t *carry);
+int mwc_host_init(uint32_t seed, int nth);
+void mwc_host_free();
+double mwc_host_dbl(int thread);
+int mwc_seed(glu_gfix_ctx *c, uint32_t seed, int nth);
+int mwc_ensure(glu_gfix_ctx *c);
+int mwc_from_host(glu_gfix_ctx *c);
+int mwc_to_host(glu_gfix_ctx *c);
+int mwc_sunitary(glu_gfix_ctx *c, void *G, long long Vp, long long n, const std::vector<long long> &first,
+                 bool reunit);
+int mwc_random_transform(glu_gfix_ctx *c);

@@ -68,53 +68,13 @@ void unpack(struct site *lat, const std::vector<double> &h) {
 }
 
 // ------------------------------------------------------------------------
-// GLU's parallel MWC_4096 generator and the random gauge transform used by
-// the Landau restarts (Utils/par_rng.c:43-186, Utils/par_MWC_4096.c:33-112,
-// Geometry/random_config.c:108-120).  One stream per GLU thread, seeded
-// SEED + thread (SEED 0: /dev/urandom), warmed up by 10000 draws, each
-// thread drawing its static block of sites.
+// The random gauge transform of the Landau restarts (Utils/par_rng.c:180-186,
+// Geometry/random_config.c:108-120), drawn from GLU's process stream
+// (mwc.hip: one MWC_4096 stream per GLU thread, seeded SEED + thread, each
+// thread drawing its static block of sites).
 // ------------------------------------------------------------------------
-struct Mwc {
-  std::vector<uint32_t> x, carry;
-  bool init = false;
-} g_rng;
-
-uint32_t mwc_next(int t) {
-  const uint64_t p = 18782ULL * (uint64_t)g_rng.x[t] + g_rng.carry[t];
-  g_rng.carry[t] = (uint32_t)(p >> 32);
-  uint32_t v = (uint32_t)p + g_rng.carry[t];
-  if (v < g_rng.carry[t]) {
-    v++;
-    g_rng.carry[t]++;
-  }
-  g_rng.x[t] = 0xfffffffeu - v;
-  return g_rng.x[t];
-}
-double mwc_dbl(int t) { return (double)(mwc_next(t) * 2.3283064365386963e-10); }
-
-void rng_init() {
-  if (g_rng.init) return;
-  const int nth = Latt.Nthreads ? (int)Latt.Nthreads : 1;
-  uint32_t seed = Latt.Seed[0];
-  if (seed == 0) {
-    FILE *f = fopen("/dev/urandom", "r");
-    if (f) {
-      if (fread(&seed, sizeof(seed), 1, f) != 1) seed = 1;
-      fclose(f);
-    }
-  }
-  g_rng.x.resize(nth);
-  g_rng.carry.resize(nth);
-  std::vector<uint32_t> tab(4096);
-  for (int t = 0; t < nth; t++) {
-    tab[0] = seed + (uint32_t)t;
-    for (uint32_t j = 1; j < 4096; j++) tab[j] = (uint32_t)(1812433253UL * (tab[j - 1] ^ (tab[j - 1] >> 30)) + j);
-    g_rng.x[t] = tab[0];
-    g_rng.carry[t] = tab[4095] % 809430660UL;
-    for (int j = 0; j < 10000; j++) mwc_dbl(t);
-  }
-  g_rng.init = true;
-}
+int rng_init() { return mwc_host_init(Latt.Seed[0], Latt.Nthreads ? (int)Latt.Nthreads : 1); }
+double mwc_dbl(int t) { return mwc_host_dbl(t); }
 
 cd polar_box(int t) {
   for (;;) {
@@ -159,10 +119,10 @@ void gtrans_host(const cd *a, cd *b, const cd *c) {
   memcpy(b, o, sizeof(o));
 }
 
-void random_transform(std::vector<double> &h) {
-  rng_init();
+int random_transform(std::vector<double> &h) {
+  if (rng_init()) return -1;
   const size_t V = volume();
-  const int nth = (int)g_rng.x.size();
+  const int nth = (int)g_mwc.x.size();
   std::vector<cd> G(V * 9);
   for (int t = 0; t < nth; t++) {  // libgomp static blocks
     size_t q = V / nth, r = V % nth;
@@ -188,6 +148,7 @@ void random_transform(std::vector<double> &h) {
       gtrans_host(&G[i * 9], (cd *)&h[(i * 4 + mu) * 18], &G[f * 9]);
     }
   }
+  return 0;
 }
 
 // ------------------------------------------------------------------------
@@ -395,9 +356,14 @@ size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int m
   pack(lat, h);
   size_t iters = 0;
   const int lux = callback ? luxury_copies() : 0;
-  const uint64_t seed = Latt.Seed[0] ? (uint64_t)Latt.Seed[0] : 1;
+  // the copies draw from GLU's process stream, continued on the device
+  if (lux && (rng_init() || mwc_from_host(c))) {
+    fprintf(stderr, "[GF] random stream unavailable: %s\n", glu_gfix_last_error());
+    return 123456789;
+  }
   if (glu_gfix_set_links(c, h.data()) ||
-      (lux ? glu_gfix_luxury(c, method, Latt.gf_alpha, acc, max_iters, lux, seed, th, &iters, nullptr)
+      (lux ? glu_gfix_luxury(c, method, Latt.gf_alpha, acc, max_iters, lux, 0, th, &iters, nullptr) ||
+                 mwc_to_host(c)
            : glu_gfix_run(c, method, Latt.gf_alpha, acc, max_iters, th, &iters))) {
     fprintf(stderr, "[GF] device gauge fixing failed: %s\n", glu_gfix_last_error());
     return 123456789;
@@ -434,7 +400,10 @@ extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t m
         printf("[IO] Something funky happened when trying to read in config again!\n");
         break;
       }
-      random_transform(h);
+      if (random_transform(h)) {
+        fprintf(stderr, "[GF] random transform failed: %s\n", glu_gfix_last_error());
+        break;
+      }
       unpack(lat, h);
       const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method, true);
       if (loc != 123456789) {
@@ -470,8 +439,13 @@ extern "C" size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW, const do
   pack(lat, h);
   size_t iters = 0;
   double th = 0.0;
-  if (glu_gfix_set_links(c, h.data()) ||
-      glu_gfix_run(c, facg == GLU_TRUE ? GLU_GFIX_CG : GLU_GFIX_SD, Latt.gf_alpha, accuracy, max_iter, &th, &iters)) {
+  // initialise_par_rng on entry, free_par_rng on return (Gfix/CFACG.c:539,
+  // 630): the slice restarts continue GLU's process stream on the device
+  const int rc = rng_init() || mwc_from_host(c) || glu_gfix_set_links(c, h.data()) ||
+                 glu_gfix_run(c, facg == GLU_TRUE ? GLU_GFIX_CG : GLU_GFIX_SD, Latt.gf_alpha, accuracy, max_iter,
+                              &th, &iters);
+  mwc_host_free();
+  if (rc) {
     fprintf(stderr, "[GF] device Coulomb gauge fixing failed: %s\n", glu_gfix_last_error());
     return 0;
   }

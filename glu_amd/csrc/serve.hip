@@ -218,9 +218,14 @@ int after_run(Request &r) {
   }
   if (++r.attempt >= 7)
     return gfix_fail(-5, "no convergence after %d random restarts (GF_TUNE %g)", r.attempt - 1, r.gf);
-  // restart from the request's input, which stays untouched until copy_out
+  // restart from the request's input, which stays untouched until copy_out,
+  // under random_transform drawn from GLU's streams, seeded with the
+  // request's seed at its first restart as glu_gfix_landau does (a pooled
+  // context's streams never carry over from another request)
   launch_pack<double>(r.links, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
-  if (int rc = glu_gfix_random_transform(c, r.restart_seed + (uint64_t)r.attempt)) return rc;
+  if (r.attempt == 1)
+    if (int rc = mwc_seed(c, (uint32_t)(r.restart_seed ? r.restart_seed : 1), c->rng_threads)) return rc;
+  if (int rc = mwc_random_transform(c)) return rc;
   if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
   if (c->run.finished) return after_run(r);
   return 1;

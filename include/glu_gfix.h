@@ -151,10 +151,14 @@ int glu_gfix_run(glu_gfix_ctx *ctx, int method, double gf_tune,
    failure) is repeated from the input links after a random gauge transform,
    at most 6 times (GF_GLU_FAILURES = 7, Headers/GLU_definitions.h:400-402);
    failed attempts add max_iters to *iters, as GLU counts them.  GLU
-   re-reads the config file and uses its own generator for the transform;
-   here restart k applies glu_gfix_random_transform(ctx, restart_seed + k)
-   to a device copy of the input.  *failures (may be NULL): restarts taken.
-   Returns -5 when all 7 attempts fail (GLU: GLU_FAILURE). */
+   re-reads the config file and applies random_transform
+   (Geometry/random_config.c:108-120) drawn from its MWC_4096 streams; here
+   each restart applies the same transform, drawn from the context's copy
+   of those streams (glu_gfix_rng_seed), to a device copy of the input.
+   restart_seed != 0 seeds the streams with SEED = restart_seed (its low
+   32 bits) at the first restart, as a GLU process starting with that SEED
+   would; 0 continues the context's streams.  *failures (may be NULL):
+   restarts taken.  Returns -5 when all 7 attempts fail (GLU: GLU_FAILURE). */
 int glu_gfix_landau(glu_gfix_ctx *ctx, int method, double gf_tune,
                     double accuracy, size_t max_iters, uint64_t restart_seed,
                     double *theta, size_t *iters, int *failures);
@@ -163,10 +167,11 @@ int glu_gfix_landau(glu_gfix_ctx *ctx, int method, double gf_tune,
    BEST_COPY): `copies` random gauge copies of the device links, each
    fixed to 1e-6 in at most 1000 iterations; the copy with the smallest
    functional among those that converged is then fixed to `accuracy`.
-   *iters = last copy's iterations + the final run's, as GLU counts.  Copy
-   k applies glu_gfix_random_transform(ctx, seed + k).  *best_copy (may be
-   NULL): the chosen copy, -1 if none qualified (the final run then starts
-   from the input). */
+   *iters = last copy's iterations + the final run's, as GLU counts.  Each
+   copy's random transform draws from the context's MWC_4096 streams,
+   seeded with SEED = seed at the start (seed 0: continuing them), as GLU's
+   copies draw from its streams.  *best_copy (may be NULL): the chosen copy,
+   -1 if none qualified (the final run then starts from the input). */
 int glu_gfix_luxury(glu_gfix_ctx *ctx, int method, double gf_tune,
                     double accuracy, size_t max_iters, int copies,
                     uint64_t seed, double *theta, size_t *iters,
@@ -281,6 +286,28 @@ int glu_gfix_stage_egauge(glu_gfix_ctx *ctx, const double *fields,
 int glu_gfix_heatbath(glu_gfix_ctx *ctx, double beta, int sweeps, int nor,
                       uint64_t seed, int cold_start);
 int glu_gfix_random_transform(glu_gfix_ctx *ctx, uint64_t seed);
+
+/* GLU's random numbers.  GLU draws from one MWC_4096 stream per OpenMP
+   thread (Utils/par_rng.c:43-186, Utils/par_MWC_4096.c:33-112): stream t is
+   seeded with SEED + t and warmed up by 10000 draws, and a loop over n sites
+   gives stream t libgomp's static block of them.  A context keeps a device
+   copy of those streams; glu_gfix_rng_seed is initialise_par_rng with
+   Latt.Seed = seed (nonzero) and Latt.Nthreads = nthreads (the result of
+   every draw depends on both, as GLU's does).  The default, used when a
+   run needs random numbers before any seeding, is seed 1 and 256 streams.
+   Coulomb runs reseed from that seed on every call (GLU frees its streams
+   at the end of Coulomb_FA, Gfix/CFACG.c:630). */
+int glu_gfix_rng_seed(glu_gfix_ctx *ctx, uint32_t seed, int nthreads);
+/* random_transform of Geometry/random_config.c:108-120 on the device links:
+   Sunitary_gen (Matrix_Ops/gramschmidt_SSE.c:175-181: nine complex
+   Gaussians of the polar method, Gram-Schmidt) per site from the context's
+   streams, then U_mu(x) <- g(x) U_mu(x) g(x + mu)^dag.  The streams must be
+   seeded. */
+int glu_gfix_glu_random_transform(glu_gfix_ctx *ctx);
+/* The first n doubles par_rng_dbl(thread) returns after GLU's seeding with
+   (seed, nthreads): the host restatement of the streams the device copies,
+   for checking them against GLU's. */
+int glu_gfix_rng_draws(uint32_t seed, int nthreads, int thread, size_t n, double *out);
 
 /* Per-kernel timing with HIP events on the context's stream.  Kernel
    classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
