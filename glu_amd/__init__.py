@@ -36,11 +36,11 @@ EXPORTS = [
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
     "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional", "glu_gfix_rng_seed", "glu_gfix_glu_random_transform",
-    "glu_gfix_rng_draws",
+    "glu_gfix_rng_draws", "glu_gfix_phase_ms", "glu_gfix_server_submit_glu",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
-           "gluon_prop"]
+           "gluon_prop", "c_deriv", "c_pr_sums", "c_cg_update", "c_line_search", "c_slice_update"]
 CUTS = {"hypercubic": 0, "psq": 1, "cylinder": 2, "conical": 3}
 
 
@@ -104,6 +104,9 @@ def lib():
             "glu_gfix_rng_seed": (C.c_int, [P, C.c_uint32, C.c_int]),
             "glu_gfix_glu_random_transform": (C.c_int, [P]),
             "glu_gfix_rng_draws": (C.c_int, [C.c_uint32, C.c_int, C.c_int, SZ, DP]),
+            "glu_gfix_phase_ms": (C.c_int, [P, DP]),
+            "glu_gfix_server_submit_glu": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_char_p, C.c_void_p,
+                                                     C.POINTER(C.c_int)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -256,6 +259,12 @@ class Gfix:
     def set_switch_theta(self, theta: float):
         _check(lib().glu_gfix_set_switch_theta(self._h, theta), "set_switch_theta")
 
+    def phase_ms(self):
+        """(FP32 phase, switch, FP64 phase) host milliseconds of the last run."""
+        ms = np.zeros(3)
+        _check(lib().glu_gfix_phase_ms(self._h, _dp(ms)), "phase_ms")
+        return tuple(float(x) for x in ms)
+
     def mixed_split(self):
         a, b = SZ(), SZ()
         _check(lib().glu_gfix_mixed_split(self._h, C.byref(a), C.byref(b)), "mixed_split")
@@ -354,6 +363,14 @@ class Server:
         t = C.c_int()
         _check(lib().glu_gfix_server_submit(self._h, (SZ * 4)(*dims), gftype, precision, method, gf_tune,
                                             accuracy, max_iters, C.c_void_p(ptr), C.byref(t)), "server_submit")
+        return t.value
+
+    def submit_glu(self, dims, ptr: int, options: str, precision=MIXED) -> int:
+        """Submit with GLU input-file keys (GFTYPE, ACCURACY, MAX_ITERS,
+        GF_TUNE, SEED, RANDOM_TRANSFORM) given as KEY = VALUE lines."""
+        t = C.c_int()
+        _check(lib().glu_gfix_server_submit_glu(self._h, (SZ * 4)(*dims), precision, options.encode(),
+                                                C.c_void_p(ptr), C.byref(t)), "server_submit_glu")
         return t.value
 
     def wait(self, ticket: int):

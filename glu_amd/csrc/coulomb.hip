@@ -454,32 +454,32 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     // pointer swap makes in the in_old of the first CG iteration.  The CG
     // state lives within this call (every call starts with an SD step), so
     // a slice that stopped may keep its fields in either buffer.
-    hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc);
+    GFIX_PROF(c, PK_CDERIV, hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc));
     enq_fa<double>(c, F, nullptr, sc.done);
-    hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs);
+    GFIX_PROF(c, PK_CPR, hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs));
     hipLaunchKernelGGL(k_cfin_pr, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0);
-    hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, F, part, g, sc, cs->nbs, 1);
+    GFIX_PROF(c, PK_CLINE, hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, F, part, g, sc, cs->nbs, 1));
     hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 1, 0);
-    hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc);
-    hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 0);
+    GFIX_PROF(c, PK_CEXP, hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc));
+    GFIX_PROF(c, PK_CUPD, hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 0));
     std::swap(F, O);
   }
   for (;;) {
     for (int b = 0; b < 8; b++) {
-      hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc);
+      GFIX_PROF(c, PK_CDERIV, hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc));
       enq_fa<double>(c, F, nullptr, sc.done);
       if (method == GLU_GFIX_CG) {
-        hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, O, part, g, sc, cs->nbs);
+        GFIX_PROF(c, PK_CPR, hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, O, part, g, sc, cs->nbs));
         hipLaunchKernelGGL(k_cfin_pr, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 1);
-        hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 1);
-        hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, sn, part, g, sc, cs->nbs, 0);
+        GFIX_PROF(c, PK_CUPD, hipLaunchKernelGGL(k_cupdate, grid, blk, 0, s, F, sn, g, sc, 1));
+        GFIX_PROF(c, PK_CLINE, hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, sn, part, g, sc, cs->nbs, 0));
         hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0, 0);
-        hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, sn, g, sc);
+        GFIX_PROF(c, PK_CEXP, hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, sn, g, sc));
         std::swap(F, O);
       } else {
         hipLaunchKernelGGL(k_cfin_ls, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0, 1);
-        hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc);
-        hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs);
+        GFIX_PROF(c, PK_CEXP, hipLaunchKernelGGL(k_cexp, grid, blk, 0, s, G, F, g, sc));
+        GFIX_PROF(c, PK_CPR, hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs));
         hipLaunchKernelGGL(k_cfin_der, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc);
       }
       hipLaunchKernelGGL(k_citer_end, dim3(1), dim3(256), 0, s, sc);
@@ -487,6 +487,7 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     if (hipMemcpyAsync(done.data(), sc.done, T * sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
       return gfix_fail(-2, "Coulomb iteration failed: %s", hipGetErrorString(hipGetLastError()));
+    if (c->prof) prof_harvest(c);
     bool all = true;
     for (int t = 0; t < T; t++) all = all && done[t];
     if (all) break;

@@ -1,6 +1,7 @@
 // gfix_api.hip -- host side of the plain-pointer C ABI: device contexts,
 // the FACG / FASD iteration drivers (Gfix/FACG.c:181-446) with device-side
 // scalars, and the mixed-precision FP32 -> FP64 schedule.
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -87,11 +88,45 @@ extern "C" size_t glu_gfix_footprint(const size_t dims[4], int gftype, int preci
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// The FFT axis descriptors and p^2 data of a context whose dims, geometry
+// and gftype are set: fills c->ax[mu] for the transformed axes and c->ps,
+// leaving the device table pointers to the caller, and writes the host
+// tables: tw[mu] (dims[mu] twiddles exp(-2 pi i k / N) for mu < D) and cosv
+// (the cos(2 pi n / L_mu) of every axis, back to back: Latt.twiddles,
+// Geometry/init.c:69-71).  The extents were checked by check_extents.
+void fft_plan(glu_gfix_ctx *c, double2 *const tw[4], double *cosv) {
+  const int D = c->gftype == GLU_GFIX_LANDAU ? 4 : 3;
+  const long long V = c->g.V;
+  int off = 0;
+  for (int mu = 0; mu < 4; mu++) {
+    const int N = c->g.L[mu];
+    c->ps.off[mu] = off;
+    c->ps.L[mu] = N;
+    const double twd = 6.283185307179586 / (double)N;
+    for (int n = 0; n < N; n++) cosv[off + n] = cos(n * twd);
+    off += N;
+    if (mu >= D) continue;
+    FftAxis ax{};
+    ax.N = N;
+    ax.nst = factor_radices(N, ax.rad);
+    ax.S = c->g.S[mu];
+    ax.npencil = (D == 4 ? V : c->g.Lcu) / N;  // per transform (Coulomb: per slice)
+    ax.P = (int)std::min<size_t>(512, pow2_floor(2048 / (size_t)N));
+    for (int k = 0; k < N; k++) {
+      const double a = 6.283185307179586 * (double)k / (double)N;
+      tw[mu][k] = make_double2(cos(a), -sin(a));
+    }
+    c->ax[mu] = ax;
+  }
+  c->ps.D = D;
+  c->ps.max = D == 4 ? 4.0 * 4 / (double)V : 4.0 * 3 / (double)c->g.Lcu;
+}
+
 // The transformed axes (x, y, z, and t for Landau) go through the LDS FFT:
 // a workgroup holds P >= 1 pencils of N points (N P <= 2048) and a stage
 // runs at most 1024 butterflies, so N is limited to 2048 and must factor
 // into the radices 2, 3, 5 and 7.
-static int check_extents(const size_t dims[4], int gftype) {
+int check_extents(const size_t dims[4], int gftype) {
   const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
   long double vol = 1;
   for (int mu = 0; mu < 4; mu++) {
@@ -130,6 +165,8 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
   c->g.Lcu = c->g.S[3];
   geo_set_div(c->g);
   c->g.xcd = 0;
+  c->ls_tile = ls_tile_fits(c->g, 5) ? 5 : 0;
+  if (const char *e = getenv("GLU_GFIX_LS_TILE")) c->ls_tile = ls_tile_fits(c->g, atoi(e)) ? atoi(e) : 0;
   if (const char *e = getenv("GLU_GFIX_XCD")) {
     c->ord_eg = c->ord_ls = std::min(std::max(atoi(e), 0), 2);
     c->ord_cl = c->ord_eg ? 1 : 0;
@@ -180,39 +217,24 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     }
     // FFT axes, twiddles and the p^2 cos tables
     const int D = gftype == GLU_GFIX_LANDAU ? 4 : 3;
-    std::vector<double> cosv;
-    for (int mu = 0; mu < 4; mu++) {
-      const int N = (int)dims[mu];
-      c->ps.off[mu] = (int)cosv.size();
-      c->ps.L[mu] = N;
-      const double twd = 6.283185307179586 / (double)N;  // Latt.twiddles, Geometry/init.c:69-71
-      for (int n = 0; n < N; n++) cosv.push_back(cos(n * twd));
-      if (mu >= D) continue;
-      FftAxis ax{};
-      ax.N = N;
-      ax.nst = factor_radices(N, ax.rad);
-      if (ax.nst < 0) {
-        rc = gfix_fail(-1, "lattice extent %d: extents must factor into 2, 3, 5 and 7", N);
-        goto bad;
-      }
-      ax.S = c->g.S[mu];
-      ax.npencil = (D == 4 ? V : c->g.Lcu) / N;  // per transform (Coulomb: per slice)
-      ax.P = (int)std::min<size_t>(512, pow2_floor(2048 / (size_t)N));
-      std::vector<double2> tw(N);
-      for (int k = 0; k < N; k++) {
-        const double a = 6.283185307179586 * (double)k / (double)N;
-        tw[k] = make_double2(cos(a), -sin(a));
-      }
-      CK(hipMalloc(&c->tw[mu], N * sizeof(double2)));
-      CK(hipMemcpy(c->tw[mu], tw.data(), N * sizeof(double2), hipMemcpyHostToDevice));
-      ax.tw = c->tw[mu];
-      c->ax[mu] = ax;
+    size_t ncos = 0;
+    for (int mu = 0; mu < 4; mu++) ncos += dims[mu];
+    std::vector<double> cosv(ncos);
+    std::vector<double2> tw[4];
+    double2 *twp[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int mu = 0; mu < D; mu++) {
+      tw[mu].resize(dims[mu]);
+      twp[mu] = tw[mu].data();
     }
-    CK(hipMalloc(&c->cosv, cosv.size() * sizeof(double)));
-    CK(hipMemcpy(c->cosv, cosv.data(), cosv.size() * sizeof(double), hipMemcpyHostToDevice));
-    c->ps.D = D;
+    fft_plan(c, twp, cosv.data());
+    for (int mu = 0; mu < D; mu++) {
+      CK(hipMalloc(&c->tw[mu], dims[mu] * sizeof(double2)));
+      CK(hipMemcpy(c->tw[mu], twp[mu], dims[mu] * sizeof(double2), hipMemcpyHostToDevice));
+      c->ax[mu].tw = c->tw[mu];
+    }
+    CK(hipMalloc(&c->cosv, ncos * sizeof(double)));
+    CK(hipMemcpy(c->cosv, cosv.data(), ncos * sizeof(double), hipMemcpyHostToDevice));
     c->ps.cosv = c->cosv;
-    c->ps.max = D == 4 ? 4.0 * 4 / (double)V : 4.0 * 3 / (double)c->g.Lcu;
     if (gftype == GLU_GFIX_COULOMB) {
       rc = coulomb_alloc(c);
       if (rc) goto bad;
@@ -432,7 +454,7 @@ template <typename T> static void enq_sd_start(glu_gfix_ctx *c, const Bufs &b) {
   launch_deriv<T>(b.U, b.F, c->part, geo_order(c, c->ord_eg), c->ctrl, s);
   launch_fin_theta(c->part, c->nblk, c->ctrl, landau_norm(c), s);
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
-  launch_line_search<T>(b.U, b.F, c->part, geo_order(c, c->ord_ls), c->ctrl, 1, s);
+  launch_line_search<T>(b.U, b.F, c->part, geo_order(c, c->ord_ls), c->ctrl, 1, s, c->ls_tile);
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 1, 0.15, 0.30, s);
   launch_copy<T>(b.F, b.sn, c->g.V, c->ctrl, s);
   launch_pr_sums<T>(b.F, b.F, c->part, c->g.V, c->ctrl, s);
@@ -455,7 +477,8 @@ template <typename T> static void enq_cg_iter(glu_gfix_ctx *c, const Bufs &b) {
   GFIX_PROF(c, pk<T>(PK_PR), launch_pr_sums<T>(b.F, b.old, c->part, c->g.V, c->ctrl, s));
   launch_fin_pr(c->part, c->nblk, c->ctrl, s);
   GFIX_PROF(c, pk<T>(PK_CGUPD), launch_cg_update<T>(b.F, b.sn, c->g.V, c->ctrl, s));
-  GFIX_PROF(c, pk<T>(PK_LSEARCH), launch_line_search<T>(b.U, b.sn, c->part, geo_order(c, c->ord_ls), c->ctrl, 0, s));
+  GFIX_PROF(c, pk<T>(PK_LSEARCH),
+            launch_line_search<T>(b.U, b.sn, c->part, geo_order(c, c->ord_ls), c->ctrl, 0, s, c->ls_tile));
   launch_fin_ls(c->part, c->nblk, c->ctrl, c->trace, 0, 0.15, 0.30, s);
   enq_update_deriv<T>(c);
   launch_fin_iter(c->part, c->nblk, c->ctrl, c->trace, landau_norm(c), s);
@@ -469,6 +492,10 @@ template <typename T> static void enq_sd_iter(glu_gfix_ctx *c, const Bufs &b) {
   enq_fa<T>(c, b.F, c->ctrl, nullptr);
   GFIX_PROF(c, pk<T>(PK_EGAUGE), launch_egauge<T>(b.U, b.F, b.G, c->g, c->ctrl, 1, s));
   launch_iter_end(c->ctrl, c->trace, 1, s);
+}
+
+static double now_ms() {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
 static int read_ctrl(glu_gfix_ctx *c) {
@@ -489,12 +516,13 @@ static int write_ctrl(glu_gfix_ctx *c) {
 template <typename T> static int theta_lin(glu_gfix_ctx *c, void *U, double *out) {
   c->deriv_ready[0] = c->deriv_ready[1] = 0;  // F and the partials are reused here
   launch_deriv<T>(U, c->F, c->part, c->g, nullptr, c->stream);
-  std::vector<double> h(c->nblk);
-  if (hipMemcpyAsync(h.data(), c->part, c->nblk * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+  // the partials' sum lands in the spare slot after the 4 nblk partials
+  double *sum = c->part + 4 * c->nblk;
+  launch_sum_part(c->part, c->nblk, sum, c->stream);
+  double s = 0.0;
+  if (hipMemcpyAsync(&s, sum, sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return gfix_fail(-2, "theta readback failed");
-  double s = 0.0;
-  for (double v : h) s += v;
   *out = 2.0 * s * landau_norm(c);
   return 0;
 }
@@ -610,8 +638,10 @@ static int enq_batch(glu_gfix_ctx *c) {
   // reported here rather than leaving the iteration silently undone
   if (const hipError_t e = hipGetLastError(); e != hipSuccess)
     return gfix_fail(-2, "iteration batch launch failed: %s", hipGetErrorString(e));
+  // without an event (a context that must not create one) the host waits
+  // on the stream instead
   if (hipMemcpyAsync(c->hctrl, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-      hipEventRecord(r.ev, c->stream) != hipSuccess)
+      (r.ev && hipEventRecord(r.ev, c->stream) != hipSuccess))
     return gfix_fail(-2, "batch enqueue failed: %s", hipGetErrorString(hipGetLastError()));
   r.inflight = 1;
   return 0;
@@ -655,6 +685,7 @@ int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_
   RunState &r = c->run;
   if (r.inflight) return gfix_fail(-1, "a run is still in flight on this context");
   r = RunState{};
+  r.t_begin = now_ms();
   r.ev = c->run_ev;
   r.method = method;
   r.gf = gf;
@@ -669,6 +700,7 @@ int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_
     r.finished = 1;
     r.theta = th0;
     r.iters = 0;
+    r.t_end = r.t_fp32_end = r.t_switched = now_ms();
     return 0;
   }
   r.mixed = c->precision == GLU_GFIX_MIXED && c->switch_theta > acc;
@@ -692,7 +724,8 @@ int landau_advance(glu_gfix_ctx *c, int block) {
   RunState &r = c->run;
   if (r.finished) return 1;
   if (r.inflight) {
-    const hipError_t e = block ? hipEventSynchronize(r.ev) : hipEventQuery(r.ev);
+    const hipError_t e = r.ev ? (block ? hipEventSynchronize(r.ev) : hipEventQuery(r.ev))
+                              : (block ? hipStreamSynchronize(c->stream) : hipStreamQuery(c->stream));
     if (e == hipErrorNotReady) return 0;
     if (e != hipSuccess) return gfix_fail(-2, "iteration batch failed: %s", hipGetErrorString(e));
     r.inflight = 0;
@@ -718,11 +751,13 @@ int landau_advance(glu_gfix_ctx *c, int block) {
     // FP32 -> FP64: replay the accumulated transform on the FP64 links and
     // carry the CG state when the links agree with where FP32 stopped
     c->last_fp32_iters = (size_t)h.total;
+    r.t_fp32_end = now_ms();
     launch_apply_gauge(c->U64, c->G64, c->g, c->stream);
     launch_promote(c->sn32, c->sn, c->g.V * 4, c->stream);
     launch_promote(c->old32, c->old, c->g.V * 4, c->stream);
     double thl = 0.0;
-    if (int rc = theta_lin<double>(c, c->U64, &thl)) return rc;
+    if (int rc = theta_lin<double>(c, c->U64, &thl)) return rc;  // waits for the switch's work
+    r.t_switched = now_ms();
     // a stalled FP32 phase left an uphill direction behind: restart CG
     const int keep = h.switched == 1 && 0.5 * thl < 10.0 * theta;
     const size_t left = r.max_iters > c->last_fp32_iters ? r.max_iters - c->last_fp32_iters : 1;
@@ -756,6 +791,8 @@ int landau_advance(glu_gfix_ctx *c, int block) {
   }
   r.theta = theta;
   r.finished = 1;
+  r.t_end = now_ms();
+  if (r.t_fp32_end == 0) r.t_fp32_end = r.t_switched = r.phase == PH_FP32 ? r.t_end : r.t_begin;
   return 1;
 }
 
@@ -899,6 +936,16 @@ extern "C" int glu_gfix_trace(glu_gfix_ctx *c, double *rows, size_t cap, size_t 
   return 0;
 }
 
+extern "C" int glu_gfix_phase_ms(glu_gfix_ctx *c, double ms[3]) {
+  if (!c || !ms) return gfix_fail(-1, "null argument");
+  const RunState &r = c->run;
+  if (!r.finished) return gfix_fail(-1, "no finished run on this context");
+  ms[0] = r.t_fp32_end - r.t_begin;
+  ms[1] = r.t_switched - r.t_fp32_end;
+  ms[2] = r.t_end - r.t_switched;
+  return 0;
+}
+
 extern "C" int glu_gfix_mixed_split(glu_gfix_ctx *c, size_t *a, size_t *b) {
   if (!c || !a || !b) return gfix_fail(-1, "null argument");
   *a = c->last_fp32_iters;
@@ -992,7 +1039,7 @@ extern "C" int glu_gfix_stage_line_search(glu_gfix_ctx *c, const double *fields,
   if (c->gftype != GLU_GFIX_LANDAU) return gfix_fail(-1, "stage entry points are Landau-only");
   if (int rc = ctrl_live(c, 0.0, 1.0)) return rc;
   if (int rc = fields_in(c, fields, c->sn)) return rc;
-  launch_line_search<double>(c->U64, c->sn, c->part, c->g, c->ctrl, 1, c->stream);
+  launch_line_search<double>(c->U64, c->sn, c->part, c->g, c->ctrl, 1, c->stream, c->ls_tile);
   launch_fin_ls(c->part, c->nblk, c->ctrl, nullptr, 1, 0.15, 0.30, c->stream);
   if (int rc = read_ctrl(c)) return rc;
   for (int k = 0; k < 3; k++) probes[k] = c->hctrl->probe[k];

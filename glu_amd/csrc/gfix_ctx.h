@@ -10,7 +10,11 @@
 struct CoulombState;
 
 // per-kernel event timing (glu_gfix_profile): kernel classes
-enum ProfKernel { PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_EGDERIV, PK_LIE, PK_PROP, PK_COUNT };
+enum ProfKernel {
+  PK_DERIV = 0, PK_FFT, PK_PR, PK_CGUPD, PK_LSEARCH, PK_EGAUGE, PK_EGDERIV, PK_LIE, PK_PROP,
+  // Coulomb slice kernels
+  PK_CDERIV, PK_CPR, PK_CUPD, PK_CLINE, PK_CEXP, PK_COUNT
+};
 struct ProfPending {
   int id;
   hipEvent_t a, b;
@@ -29,6 +33,9 @@ struct RunState {
   hipEvent_t ev = nullptr;
   size_t iters = 0;
   double theta = 0.0;
+  // host clock (ms since the run began) when the FP32 phase was seen to
+  // end, when the switch's work was done, and when the run finished
+  double t_begin = 0, t_fp32_end = 0, t_switched = 0, t_end = 0;
 };
 
 struct glu_gfix_ctx {
@@ -72,6 +79,9 @@ struct glu_gfix_ctx {
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner
   int ord_eg = 1, ord_ls = 1;
+  // line search on 256-site tiles (kLsTileShape[ls_tile - 1]; 0: one site
+  // per thread); set at creation, then by the autotuner
+  int ls_tile = 0;
   int ord_cl = 1;  // Coulomb slice stencils: XCD-contiguous (1) or plain (0)
   int use_graph = 0;  // replay iteration batches from captured HIP graphs
   // iterations enqueued between host checks; the iterations enqueued past
@@ -158,6 +168,8 @@ extern "C" int glu_gfix_reunit(glu_gfix_ctx *c);
 // staging buffer (kChunk * 72 doubles, allocated on first use)
 constexpr long long kChunk = 1 << 17;
 int ensure_stage(glu_gfix_ctx *c);
+int check_extents(const size_t dims[4], int gftype);
+void fft_plan(glu_gfix_ctx *c, double2 *const tw[4], double *cosv);
 void glueprop_free(glu_gfix_ctx *c);
 
 // GLU's MWC_4096 streams (mwc.hip)

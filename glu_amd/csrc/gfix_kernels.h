@@ -52,9 +52,17 @@ template <typename T>
 void launch_cg_update(const void *F, void *sn, long long V, const Ctrl *ctrl, hipStream_t s);
 template <typename T>
 void launch_copy(const void *F, void *a, long long V, const Ctrl *ctrl, hipStream_t s);
+// the line search on 256-site tiles: variant k (1..kLsTiles) has the tile
+// kLsTileShape[k - 1] = (TX, TY, TZ, TT), the link columns held in
+// registers for both probes (k <= 4) or loaded per probe (k > 4); 0, or a
+// shape that does not divide the lattice, runs the one-site-per-thread kernel
+constexpr int kLsTiles = 8;
+constexpr int kLsTileShape[kLsTiles][4] = {{8, 4, 4, 2}, {4, 4, 4, 4}, {8, 8, 2, 2}, {16, 4, 2, 2},
+                                           {8, 4, 4, 2}, {4, 4, 4, 4}, {8, 8, 2, 2}, {16, 4, 2, 2}};
+bool ls_tile_fits(const Geo &g, int tile);
 template <typename T>
 void launch_line_search(const void *U, const void *F, double *part, const Geo &g, const Ctrl *ctrl, int force,
-                        hipStream_t s);
+                        hipStream_t s, int tile = 0);
 template <typename T>
 void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s);
 template <typename T>
@@ -72,6 +80,7 @@ void launch_fin_sd(const double *part, long long nb, Ctrl *ctrl, hipStream_t s);
 void launch_fin_ls(const double *part, long long nb, Ctrl *ctrl, double *trace, int force, double pl1, double pl2,
                    hipStream_t s);
 void launch_iter_end(Ctrl *ctrl, double *trace, int fasd, hipStream_t s);
+void launch_sum_part(const double *part, long long nb, double *out, hipStream_t s);
 void launch_fin_iter(const double *part, long long nb, Ctrl *ctrl, double *trace, double norm, hipStream_t s);
 void launch_begin_segment(Ctrl *ctrl, int sd, hipStream_t s);
 void launch_demote(const void *U, void *W, long long n, hipStream_t s);

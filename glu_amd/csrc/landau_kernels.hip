@@ -185,6 +185,167 @@ __global__ __launch_bounds__(kBlock) void k_line_search(const vec2<T> *__restric
 }
 
 // ---------------------------------------------------------------------------
+// The same probes with each probe matrix built once.  A workgroup owns a
+// TX x TY x TZ x TT tile of 256 sites and builds, per probe, the matrices
+// g(x; alpha) of its sites and of the forward faces of the tile (the
+// neighbours x + mu outside it) into LDS: 2 + 256 / TX + ... builds per
+// 256 sites instead of 5 per site.  Only columns 0 and 1 are kept: with
+// M = A u (columns 0, 1 from u's stored columns) and B the neighbour's
+// matrix, both in SU(3) so that their third columns are conj(c0 x c1),
+//   Re tr(A u B^dag) = Re q00 + Re q11 + Re(q00 q11 - q01 q10),
+//   q_jk = sum_r M_j[r] conj(B_k[r])
+// (the Binet-Cauchy identity for the third columns), so neither u's nor
+// B's third column is ever formed; the probe at 0 takes u's third diagonal
+// entry from the same identity.  HOLD keeps a site's link columns in
+// registers for both probes; without it they are loaded again for the
+// second probe (from the caches), which frees registers for more waves.
+// The arithmetic is FP64, as in k_line_search.
+// ---------------------------------------------------------------------------
+template <typename T, int TX, int TY, int TZ, int TT, bool HOLD>
+__global__ __launch_bounds__(kBlock) void k_line_search_tile(const vec2<T> *__restrict__ U,
+                                                            const vec2<T> *__restrict__ F, double *__restrict__ part,
+                                                            Geo g, const Ctrl *__restrict__ ctrl, int force) {
+  constexpr int NT = TX * TY * TZ * TT;
+  static_assert(NT == kBlock, "one site per thread");
+  constexpr int HX = TY * TZ * TT, HY = TX * TZ * TT, HZ = TX * TY * TT, HT = TX * TY * TZ;
+  constexpr int OX = NT, OY = OX + HX, OZ = OY + HY, OT = OZ + HZ, NB = OT + HT;
+  __shared__ double lds[12 * NB];  // entry e of slot j (re, im of 6 complex) at lds[e * NB + j]
+  if (ctrl->done) return;
+  if (!force && !(ctrl->theta > ctrl->cg_tol)) return;
+  const long long V = g.V;
+  const unsigned lb = logical_block(g);
+  const int ntx = g.L[0] / TX, nty = g.L[1] / TY, ntz = g.L[2] / TZ;
+  unsigned r = lb;
+  const int x0 = (int)(r % (unsigned)ntx) * TX;
+  r /= (unsigned)ntx;
+  const int y0 = (int)(r % (unsigned)nty) * TY;
+  r /= (unsigned)nty;
+  const int z0 = (int)(r % (unsigned)ntz) * TZ;
+  const int t0 = (int)(r / (unsigned)ntz) * TT;
+  const int tid = threadIdx.x;
+  const int lx = tid % TX, ly = (tid / TX) % TY, lz = (tid / (TX * TY)) % TZ, lt = tid / (TX * TY * TZ);
+  auto site = [&](int x, int y, int z, int t) -> long long {
+    return (long long)x + g.S[1] * y + g.S[2] * z + g.S[3] * t;
+  };
+  const long long me = site(x0 + lx, y0 + ly, z0 + lz, t0 + lt);
+  // LDS slot of the forward neighbour in each direction
+  int nbs[4];
+  nbs[0] = lx < TX - 1 ? tid + 1 : OX + ly + TY * (lz + TZ * lt);
+  nbs[1] = ly < TY - 1 ? tid + TX : OY + lx + TX * (lz + TZ * lt);
+  nbs[2] = lz < TZ - 1 ? tid + TX * TY : OZ + lx + TX * (ly + TY * lt);
+  nbs[3] = lt < TT - 1 ? tid + TX * TY * TZ : OT + lx + TX * (ly + TY * lz);
+  // this site's link columns (col 0 rows 0..2, col 1 rows 0..2)
+  cx<T> uh[HOLD ? kND : 1][6];
+  auto load_u = [&](int mu, cx<T> (&w)[6]) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) w[k] = ld<T>(U + ((long long)mu * 6 + k) * V + me);
+  };
+  double v[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int mu = 0; mu < kND; mu++) {
+    cx<T> w[6];
+    load_u(mu, w);
+    if constexpr (HOLD) {
+#pragma unroll
+      for (int k = 0; k < 6; k++) uh[mu][k] = w[k];
+    }
+    const double u00r = w[0].re, u00i = w[0].im, u10r = w[1].re, u10i = w[1].im;
+    const double u01r = w[3].re, u01i = w[3].im, u11r = w[4].re, u11i = w[4].im;
+    // Re U00 + Re U11 + Re U22, U22 = conj(U00 U11 - U10 U01)
+    v[0] += u00r + u11r + (u00r * u11r - u00i * u11i) - (u10r * u01r - u10i * u01i);
+  }
+#pragma unroll 1
+  for (int pr = 1; pr <= 2; pr++) {
+    const double al = pr == 1 ? 0.15 : 0.30;
+    if (pr == 2) __syncthreads();  // the first probe's matrices are no longer read
+    for (int j = tid; j < NB; j += kBlock) {
+      long long s = me;
+      if (j >= NT) {
+        int h = j - NT;
+        if (h < HX) {
+          s = site((x0 + TX) % g.L[0], y0 + h % TY, z0 + (h / TY) % TZ, t0 + h / (TY * TZ));
+        } else if ((h -= HX) < HY) {
+          s = site(x0 + h % TX, (y0 + TY) % g.L[1], z0 + (h / TX) % TZ, t0 + h / (TX * TZ));
+        } else if ((h -= HY) < HZ) {
+          s = site(x0 + h % TX, y0 + (h / TX) % TY, (z0 + TZ) % g.L[2], t0 + h / (TX * TY));
+        } else {
+          h -= HZ;
+          s = site(x0 + h % TX, y0 + (h / TX) % TY, z0 + h / (TX * TY), (t0 + TT) % g.L[3]);
+        }
+      }
+      auto f = [&](long long k) {
+        const cx<T> z = ld<T>(F + k * V + s);
+        return mk<double>((double)z.re, (double)z.im);
+      };
+      const su3<double> m = gauge_cols01<double>(f(0), f(1), f(2), f(3), al);
+      const cx<double> c[6] = {m.e[0], m.e[3], m.e[6], m.e[1], m.e[4], m.e[7]};
+#pragma unroll
+      for (int e = 0; e < 6; e++) {
+        lds[(2 * e) * NB + j] = c[e].re;
+        lds[(2 * e + 1) * NB + j] = c[e].im;
+      }
+    }
+    __syncthreads();
+    // this site's matrix A, completed: a[r][k] = A_rk
+    cx<double> a[3][3];
+#pragma unroll
+    for (int rr = 0; rr < 3; rr++)
+#pragma unroll
+      for (int k = 0; k < 2; k++)
+        a[rr][k] = mk<double>(lds[(2 * (3 * k + rr)) * NB + tid], lds[(2 * (3 * k + rr) + 1) * NB + tid]);
+    a[0][2] = cconj(csub(cmul(a[1][0], a[2][1]), cmul(a[2][0], a[1][1])));
+    a[1][2] = cconj(csub(cmul(a[2][0], a[0][1]), cmul(a[0][0], a[2][1])));
+    a[2][2] = cconj(csub(cmul(a[0][0], a[1][1]), cmul(a[1][0], a[0][1])));
+    // one direction's trace: nb is the LDS slot of the forward neighbour
+    auto trace_mu = [&](int mu, int nb) -> double {
+      cx<T> w[6];
+      if constexpr (HOLD) {
+#pragma unroll
+        for (int k = 0; k < 6; k++) w[k] = uh[mu][k];
+      } else {
+        load_u(mu, w);
+      }
+      // M = A u, columns 0 and 1
+      cx<double> M[2][3];
+#pragma unroll
+      for (int jc = 0; jc < 2; jc++) {
+        const cx<double> u0 = mk<double>((double)w[3 * jc].re, (double)w[3 * jc].im);
+        const cx<double> u1 = mk<double>((double)w[3 * jc + 1].re, (double)w[3 * jc + 1].im);
+        const cx<double> u2 = mk<double>((double)w[3 * jc + 2].re, (double)w[3 * jc + 2].im);
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+          M[jc][rr] = cadd(cadd(cmul(a[rr][0], u0), cmul(a[rr][1], u1)), cmul(a[rr][2], u2));
+      }
+      cx<double> q[2][2];
+#pragma unroll
+      for (int k = 0; k < 2; k++) {
+        cx<double> bk[3];
+#pragma unroll
+        for (int rr = 0; rr < 3; rr++)
+          bk[rr] = mk<double>(lds[(2 * (3 * k + rr)) * NB + nb], lds[(2 * (3 * k + rr) + 1) * NB + nb]);
+#pragma unroll
+        for (int jc = 0; jc < 2; jc++)
+          q[jc][k] = cadd(cadd(cmulc(M[jc][0], bk[0]), cmulc(M[jc][1], bk[1])), cmulc(M[jc][2], bk[2]));
+      }
+      const cx<double> d = csub(cmul(q[0][0], q[1][1]), cmul(q[0][1], q[1][0]));
+      return q[0][0].re + q[1][1].re + d.re;
+    };
+    double acc = 0.0;
+    if constexpr (HOLD) {
+#pragma unroll
+      for (int mu = 0; mu < kND; mu++) acc += trace_mu(mu, nbs[mu]);
+    } else {
+      // one direction at a time: fewer values live
+#pragma unroll 1
+      for (int mu = 0; mu < kND; mu++)
+        acc += trace_mu(mu, mu == 0 ? nbs[0] : mu == 1 ? nbs[1] : mu == 2 ? nbs[2] : nbs[3]);
+    }
+    v[pr] = acc;
+  }
+  block_sum_store<3>(v, part + 3 * lb);
+}
+
+// ---------------------------------------------------------------------------
 // U_mu(x) <- g(x) U_mu(x) g(x+mu)^dagger: Gfix/line_search.c:60-97.
 // With G != null the site transform is also accumulated in FP64,
 // G(x) <- g(x) G(x) with g rebuilt in FP64 from the same field values: the
@@ -447,6 +608,13 @@ __global__ void k_fin_iter(const double *__restrict__ part, long long nb, Ctrl *
   if (threadIdx.x == 0 && !iter_end_body(ctrl, trace, 0)) ctrl->theta = s[0] * norm;
 }
 
+// plain sum of nb partials into *out (fixed order)
+__global__ void k_sum_part(const double *__restrict__ part, long long nb, double *out) {
+  double s[1];
+  final_sum<1>(part, nb, s);
+  if (threadIdx.x == 0) *out = s[0];
+}
+
 __global__ void k_begin_segment(Ctrl *ctrl, int sd) {
   ctrl->done = 0;
   ctrl->iters = 0;
@@ -619,11 +787,38 @@ template <typename T> void launch_copy(const void *F, void *a, long long V, cons
   const unsigned b = (unsigned)std::min<long long>(nblk(n4), 8192);
   hipLaunchKernelGGL(k_copy<T>, dim3(b), dim3(kBlock), 0, s, (const vec2<T> *)F, (vec2<T> *)a, n4, ctrl);
 }
+template <typename T, int TX, int TY, int TZ, int TT, bool HOLD>
+static void launch_ls_tile(const void *U, const void *F, double *part, Geo g, const Ctrl *ctrl, int force,
+                           hipStream_t s) {
+  g.xcd = g.xcd ? 1 : 0;  // tiles are numbered x fastest: plain or XCD-contiguous
+  hipLaunchKernelGGL((k_line_search_tile<T, TX, TY, TZ, TT, HOLD>), dim3(nblk(g.V)), dim3(kBlock), 0, s,
+                     (const vec2<T> *)U, (const vec2<T> *)F, part, g, ctrl, force);
+}
+
+bool ls_tile_fits(const Geo &g, int tile) {
+  if (tile < 1 || tile > kLsTiles) return false;
+  const int *t = kLsTileShape[tile - 1];
+  for (int mu = 0; mu < 4; mu++)
+    if (g.L[mu] % t[mu]) return false;
+  return true;
+}
+
 template <typename T>
 void launch_line_search(const void *U, const void *F, double *part, const Geo &g, const Ctrl *ctrl, int force,
-                        hipStream_t s) {
-  hipLaunchKernelGGL(k_line_search<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)U, (const vec2<T> *)F,
-                     part, g, ctrl, force);
+                        hipStream_t s, int tile) {
+  switch (ls_tile_fits(g, tile) ? tile : 0) {
+    case 1: return launch_ls_tile<T, 8, 4, 4, 2, true>(U, F, part, g, ctrl, force, s);
+    case 2: return launch_ls_tile<T, 4, 4, 4, 4, true>(U, F, part, g, ctrl, force, s);
+    case 3: return launch_ls_tile<T, 8, 8, 2, 2, true>(U, F, part, g, ctrl, force, s);
+    case 4: return launch_ls_tile<T, 16, 4, 2, 2, true>(U, F, part, g, ctrl, force, s);
+    case 5: return launch_ls_tile<T, 8, 4, 4, 2, false>(U, F, part, g, ctrl, force, s);
+    case 6: return launch_ls_tile<T, 4, 4, 4, 4, false>(U, F, part, g, ctrl, force, s);
+    case 7: return launch_ls_tile<T, 8, 8, 2, 2, false>(U, F, part, g, ctrl, force, s);
+    case 8: return launch_ls_tile<T, 16, 4, 2, 2, false>(U, F, part, g, ctrl, force, s);
+    default:
+      hipLaunchKernelGGL(k_line_search<T>, dim3(nblk(g.V)), dim3(kBlock), 0, s, (const vec2<T> *)U,
+                         (const vec2<T> *)F, part, g, ctrl, force);
+  }
 }
 template <typename T>
 void launch_egauge(void *U, const void *F, void *G, const Geo &g, const Ctrl *ctrl, int fixed, hipStream_t s) {
@@ -654,7 +849,7 @@ template <typename T> void launch_identity(void *G, long long V, hipStream_t s) 
   template void launch_cg_update<T>(const void *, void *, long long, const Ctrl *, hipStream_t);               \
   template void launch_copy<T>(const void *, void *, long long, const Ctrl *, hipStream_t);                    \
   template void launch_line_search<T>(const void *, const void *, double *, const Geo &, const Ctrl *, int,   \
-                                      hipStream_t);                                                           \
+                                      hipStream_t, int);                                                      \
   template void launch_egauge<T>(void *, const void *, void *, const Geo &, const Ctrl *, int, hipStream_t);  \
   template void launch_egauge_deriv<T>(const void *, void *, const void *, void *, void *, void *, double *,     \
                                        const Geo &, const Ctrl *, hipStream_t);                                   \
@@ -679,6 +874,9 @@ void launch_fin_ls(const double *part, long long nb, Ctrl *ctrl, double *trace, 
 }
 void launch_fin_iter(const double *part, long long nb, Ctrl *ctrl, double *trace, double norm, hipStream_t s) {
   hipLaunchKernelGGL(k_fin_iter, dim3(1), dim3(1024), 0, s, part, nb, ctrl, trace, norm);
+}
+void launch_sum_part(const double *part, long long nb, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_part, dim3(1), dim3(1024), 0, s, part, nb, out);
 }
 void launch_iter_end(Ctrl *ctrl, double *trace, int fasd, hipStream_t s) {
   hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(1), 0, s, ctrl, trace, fasd);

@@ -14,17 +14,21 @@
 // with: every context owns its buffers and every reduction is
 // deterministic.
 //
-// Landau requests follow Landau()'s restart policy (glu_gfix_landau);
-// Coulomb requests run to completion when admitted, on the worker thread
-// (the batched-slice driver is synchronous), which delays the polling of
-// the Landau runs in flight meanwhile but not their results.
+// Landau requests follow Landau()'s restart policy (glu_gfix_landau).
+// Coulomb requests (the batched-slice driver is synchronous) run on a host
+// thread of their own, so the worker keeps advancing the Landau runs in
+// flight meanwhile.  A request may carry GLU's own input-file keys
+// (GFTYPE, ACCURACY, MAX_ITERS, GF_TUNE, SEED, RANDOM_TRANSFORM;
+// IO/input_reader.c:188-193, 444-481, 510-517).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -57,6 +61,12 @@ struct Request {
   double switch_theta = 0.0;
   int use_graph = 1;
   uint64_t restart_seed = 0;
+  // GLU's RANDOM_TRANSFORM = YES: the input is first transformed with the
+  // streams seeded SEED = seed, and the restarts continue those streams
+  int random_transform = 0;
+  uint32_t seed = 0;
+  // a Coulomb run on its own thread: 1 while it runs (worker-visible)
+  std::shared_ptr<std::atomic<int>> async;
   int attempt = 0;   // Landau() restarts taken so far
   size_t total = 0;  // iterations counted across restarts
   int rc = 0;
@@ -174,6 +184,7 @@ void complete(glu_gfix_server *s, Request &r, int rc, const std::string &err) {
 }
 
 int after_run(Request &r);
+int copy_out(Request &r);
 
 // start a request on its context: load the links, begin the run.  Returns
 // 0 and sets *finished when nothing is left to poll.
@@ -184,10 +195,27 @@ int start(glu_gfix_server *s, Request &r, bool *finished) {
     return gfix_fail(-2, "serving: event creation failed");
   c->use_graph = r.use_graph;
   if (r.switch_theta > 0.0) c->switch_theta = r.switch_theta;
+  c->mwc_live = 0;  // a pooled context's streams never carry over
+  c->rng_seed = r.seed ? r.seed : 1;
   launch_pack<double>(r.links, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
+  if (r.random_transform) {  // GLU's read_and_fix with RANDOM_TRANSFORM = YES
+    if (int rc = mwc_seed(c, c->rng_seed, c->rng_threads)) return rc;
+    if (int rc = mwc_random_transform(c)) return rc;
+  }
   if (r.gftype == GLU_GFIX_COULOMB) {
-    if (int rc = coulomb_fix(c, r.method, r.gf, r.acc, r.max_iters, &r.theta, &r.iters)) return rc;
-    *finished = true;
+    // Coulomb_FA on a thread of its own; it ends with the output copy
+    // enqueued, and the worker takes over from there
+    r.async = std::make_shared<std::atomic<int>>(1);
+    Request *rp = &r;
+    const int dev = s->device;
+    std::thread([rp, c, dev, flag = r.async] {
+      (void)hipSetDevice(dev);
+      int rc = coulomb_fix(c, rp->method, rp->gf, rp->acc, rp->max_iters, &rp->theta, &rp->iters);
+      if (!rc) rc = copy_out(*rp);
+      rp->rc = rc;
+      if (rc) rp->err = glu_gfix_last_error();
+      flag->store(0, std::memory_order_release);
+    }).detach();
     return 0;
   }
   if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
@@ -223,7 +251,7 @@ int after_run(Request &r) {
   // request's seed at its first restart as glu_gfix_landau does (a pooled
   // context's streams never carry over from another request)
   launch_pack<double>(r.links, c->U64, c->g.V, 4, 0, c->g.V, c->stream);
-  if (r.attempt == 1)
+  if (r.attempt == 1 && !r.random_transform)
     if (int rc = mwc_seed(c, (uint32_t)(r.restart_seed ? r.restart_seed : 1), c->rng_threads)) return rc;
   if (int rc = mwc_random_transform(c)) return rc;
   if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
@@ -267,7 +295,12 @@ void worker_main(glu_gfix_server *s) {
       r.state = RQ_RUNNING;
       r.switch_theta = s->switch_theta;
       r.use_graph = s->use_graph;
-      r.restart_seed = s->restart_seed;
+      if (!r.seed) {
+        r.restart_seed = s->restart_seed;
+        r.seed = (uint32_t)s->restart_seed;
+      } else {
+        r.restart_seed = r.seed;
+      }
       admitted.push_back(&r);
       s->queue.pop_front();
     }
@@ -292,6 +325,16 @@ void worker_main(glu_gfix_server *s) {
     }
     bool progressed = !admitted.empty();
     for (Request *r : active) {
+      if (r->async) {  // a Coulomb run on its own thread
+        if (r->async->load(std::memory_order_acquire)) continue;
+        const int rc = r->rc;
+        r->async.reset();
+        if (rc) {
+          ended.push_back({r, rc, r->err});
+          continue;
+        }
+        progressed = true;
+      }
       if (!r->copying) {
         const int st = landau_advance(r->ctx, 0);
         if (st < 0) {
@@ -371,9 +414,8 @@ extern "C" int glu_gfix_server_configure(glu_gfix_server *s, double switch_theta
   return 0;
 }
 
-extern "C" int glu_gfix_server_submit(glu_gfix_server *s, const size_t dims[4], int gftype, int precision,
-                                      int method, double gf_tune, double accuracy, size_t max_iters,
-                                      double *links_dev, int *ticket) {
+static int submit(glu_gfix_server *s, const size_t dims[4], int gftype, int precision, int method, double gf_tune,
+                  double accuracy, size_t max_iters, double *links_dev, int *ticket, uint32_t seed, int rtrans) {
   if (!s || !dims || !links_dev || !ticket) return gfix_fail(-1, "glu_gfix_server_submit: null argument");
   if (gftype != GLU_GFIX_LANDAU && gftype != GLU_GFIX_COULOMB) return gfix_fail(-1, "unknown gftype %d", gftype);
   if (precision != GLU_GFIX_FP64 && precision != GLU_GFIX_MIXED)
@@ -389,6 +431,8 @@ extern "C" int glu_gfix_server_submit(glu_gfix_server *s, const size_t dims[4], 
   r.acc = accuracy;
   r.max_iters = max_iters;
   r.links = links_dev;
+  r.seed = seed;
+  r.random_transform = rtrans;
   r.bytes = gfix_footprint(dims, gftype, precision);
   std::lock_guard<std::mutex> lk(s->mu);
   if (r.bytes > s->budget)
@@ -401,6 +445,73 @@ extern "C" int glu_gfix_server_submit(glu_gfix_server *s, const size_t dims[4], 
   *ticket = t;
   s->cv_work.notify_one();
   return 0;
+}
+
+extern "C" int glu_gfix_server_submit(glu_gfix_server *s, const size_t dims[4], int gftype, int precision,
+                                      int method, double gf_tune, double accuracy, size_t max_iters,
+                                      double *links_dev, int *ticket) {
+  return submit(s, dims, gftype, precision, method, gf_tune, accuracy, max_iters, links_dev, ticket, 0, 0);
+}
+
+// GLU's gauge-fixing keys from the text of an input file (KEY = VALUE
+// lines; other keys are ignored): read_gf_struct and the SEED /
+// RANDOM_TRANSFORM tags of IO/input_reader.c:188-193, 444-481, 510-517.
+// ACCURACY is the exponent: the accuracy is 10^-ACCURACY.
+extern "C" int glu_gfix_server_submit_glu(glu_gfix_server *s, const size_t dims[4], int precision,
+                                          const char *options, double *links_dev, int *ticket) {
+  if (!s || !dims || !options || !links_dev || !ticket)
+    return gfix_fail(-1, "glu_gfix_server_submit_glu: null argument");
+  std::map<std::string, std::string> kv;
+  const char *p = options;
+  while (*p) {
+    const char *e = strchr(p, '\n');
+    std::string line(p, e ? (size_t)(e - p) : strlen(p));
+    p = e ? e + 1 : p + line.size();
+    const size_t eq = line.find('=');
+    if (eq == std::string::npos) continue;
+    auto trim = [](std::string t) {
+      const size_t a = t.find_first_not_of(" \t\r"), b = t.find_last_not_of(" \t\r");
+      return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+    };
+    kv[trim(line.substr(0, eq))] = trim(line.substr(eq + 1));
+  }
+  auto need = [&](const char *k, std::string *v) {
+    auto it = kv.find(k);
+    if (it == kv.end() || it->second.empty()) return gfix_fail(-1, "[IO] tag %s not found", k);
+    *v = it->second;
+    return 0;
+  };
+  std::string gft, acc, mit, tune;
+  if (int rc = need("GFTYPE", &gft)) return rc;
+  if (int rc = need("ACCURACY", &acc)) return rc;
+  if (int rc = need("MAX_ITERS", &mit)) return rc;
+  if (int rc = need("GF_TUNE", &tune)) return rc;
+  int gftype;
+  if (gft == "LANDAU")
+    gftype = GLU_GFIX_LANDAU;
+  else if (gft == "COULOMB")
+    gftype = GLU_GFIX_COULOMB;
+  else
+    return gfix_fail(-1, "GFTYPE %s is not provided by this library (LANDAU, COULOMB)", gft.c_str());
+  const long long maxit = atoll(mit.c_str());
+  if (maxit <= 0) return gfix_fail(-1, "MAX_ITERS %s must be positive", mit.c_str());
+  unsigned seed = 0;
+  auto sit = kv.find("SEED");
+  if (sit != kv.end() && sscanf(sit->second.c_str(), "%u", &seed) != 1)
+    return gfix_fail(-1, "SEED %s is not an unsigned integer", sit->second.c_str());
+  auto rit = kv.find("RANDOM_TRANSFORM");
+  const int rtrans = rit != kv.end() && rit->second == "YES";
+  if (seed == 0 && (rtrans || sit != kv.end())) {  // SEED = 0: GLU reads the entropy pool
+    FILE *f = fopen("/dev/urandom", "r");
+    if (!f || fread(&seed, sizeof(seed), 1, f) != 1) {
+      if (f) fclose(f);
+      return gfix_fail(-1, "[RNG] /dev/urandom not read");
+    }
+    fclose(f);
+    if (!seed) seed = 1;
+  }
+  return submit(s, dims, gftype, precision, GLU_GFIX_CG, atof(tune.c_str()), pow(10.0, -atof(acc.c_str())),
+                (size_t)maxit, links_dev, ticket, seed, rtrans);
 }
 
 extern "C" int glu_gfix_server_wait(glu_gfix_server *s, int ticket, double *theta, size_t *iters,

@@ -101,9 +101,8 @@ __device__ __forceinline__ float rsqrt_nr(float x) {
   return r;
 }
 
-// Gram-Schmidt on columns 0 and 1, column 2 completed:
-// Matrix_Ops/gramschmidt.c:73-165
-template <typename T> __device__ __forceinline__ void gram_reunit(su3<T> &m) {
+// Gram-Schmidt on columns 0 and 1 (column 2 untouched)
+template <typename T> __device__ __forceinline__ void gram_cols01(su3<T> &m) {
   T n = cnorm2(m.e[0]) + cnorm2(m.e[3]) + cnorm2(m.e[6]);
   n = rsqrt_nr(n);
   m.e[0] = cscale(m.e[0], n);
@@ -119,6 +118,12 @@ template <typename T> __device__ __forceinline__ void gram_reunit(su3<T> &m) {
   m.e[1] = cscale(m.e[1], q);
   m.e[4] = cscale(m.e[4], q);
   m.e[7] = cscale(m.e[7], q);
+}
+
+// Gram-Schmidt on columns 0 and 1, column 2 completed:
+// Matrix_Ops/gramschmidt.c:73-165
+template <typename T> __device__ __forceinline__ void gram_reunit(su3<T> &m) {
+  gram_cols01(m);
   complete_col2(m);
 }
 
@@ -134,6 +139,20 @@ __device__ __forceinline__ su3<T> gauge_matrix(cx<T> f0, cx<T> f1, cx<T> f2, cx<
   g.e[6] = mk<T>(-alpha * f2.re, alpha * f2.im);
   g.e[7] = mk<T>(-alpha * f3.re, alpha * f3.im);
   gram_reunit(g);
+  return g;
+}
+
+// columns 0 and 1 of gauge_matrix (the two the third is completed from)
+template <typename T>
+__device__ __forceinline__ su3<T> gauge_cols01(cx<T> f0, cx<T> f1, cx<T> f2, cx<T> f3, T alpha) {
+  su3<T> g;
+  g.e[0] = mk<T>(T(1), alpha * f0.im);
+  g.e[1] = cscale(f1, alpha);
+  g.e[3] = mk<T>(-g.e[1].re, g.e[1].im);
+  g.e[4] = mk<T>(T(1), -alpha * f0.re);
+  g.e[6] = mk<T>(-alpha * f2.re, alpha * f2.im);
+  g.e[7] = mk<T>(-alpha * f3.re, alpha * f3.im);
+  gram_cols01(g);
   return g;
 }
 

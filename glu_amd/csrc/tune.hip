@@ -27,6 +27,7 @@ struct Tuning {
   int pv[4];
   int use_xy;
   int ord_eg, ord_ls;
+  int ls_tile;
   std::string report;
 };
 
@@ -124,10 +125,30 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
       if (eg[o] < eg[t.ord_eg]) t.ord_eg = o;
       if (ls[o] < ls[t.ord_ls]) t.ord_ls = o;
     }
-    snprintf(line, sizeof(line), "stencil order: update %s, line search %s\n", names[t.ord_eg], names[t.ord_ls]);
+    // the tiled line search (in the XCD-contiguous tile order) against the
+    // best one-site-per-thread order
+    t.ls_tile = 0;
+    float best_ls = ls[t.ord_ls];
+    for (int k = 1; k <= kLsTiles; k++) {
+      if (!ls_tile_fits(c->g, k)) continue;
+      const Geo g = geo_order(c, 1);
+      const float ms = time_ms(c, [&] { launch_line_search<T>(U, sn, c->part, g, dc, 1, c->stream, k); }, kReps);
+      const int *s = kLsTileShape[k - 1];
+      snprintf(line, sizeof(line), "line search tile %dx%dx%dx%d, links %s: %.4f ms\n", s[0], s[1], s[2], s[3],
+               k <= kLsTiles / 2 ? "held" : "reloaded", ms);
+      t.report += line;
+      if (ms < best_ls) {
+        best_ls = ms;
+        t.ls_tile = k;
+      }
+    }
+    if (t.ls_tile) t.ord_ls = 1;
+    snprintf(line, sizeof(line), "stencil order: update %s, line search %s%s\n", names[t.ord_eg], names[t.ord_ls],
+             t.ls_tile ? " (tiles)" : "");
     t.report += line;
   } else {
     t.ord_eg = t.ord_ls = 1;
+    t.ls_tile = c->ls_tile;
   }
   (void)hipStreamSynchronize(c->stream);
   if (dc) (void)hipFree(dc);
@@ -139,6 +160,7 @@ void apply(glu_gfix_ctx *c, const Tuning &t) {
   c->use_xy = t.use_xy;
   c->ord_eg = t.ord_eg;
   c->ord_ls = t.ord_ls;
+  c->ls_tile = t.ls_tile;
   for (hipGraphExec_t &gx : c->graph)  // captured launches used the old shapes
     if (gx) {
       (void)hipGraphExecDestroy(gx);

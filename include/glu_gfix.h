@@ -255,6 +255,13 @@ int glu_gfix_write_nersc(glu_gfix_ctx *ctx, const char *path, int two_rows, int 
    and the cap keeps beta bounded there.  The FP64 phase follows GLU. */
 int glu_gfix_set_switch_theta(glu_gfix_ctx *ctx, double switch_theta);
 
+/* Host-clock milliseconds of the last Landau run (of a glu_gfix_landau or
+   luxury call: its last attempt): ms[0] the FP32 phase (0 for an FP64 run),
+   ms[1] the switch (the FP64 transform of the links, the CG state
+   promoted), ms[2] the FP64 phase.  Each phase ends at the batch in which
+   it stopped, so ms[0] includes up to a batch of no-op iterations. */
+int glu_gfix_phase_ms(glu_gfix_ctx *ctx, double ms[3]);
+
 /* Iterations of the last mixed-precision run spent in FP32 / FP64. */
 int glu_gfix_mixed_split(glu_gfix_ctx *ctx, size_t *fp32_iters,
                          size_t *fp64_iters);
@@ -313,7 +320,9 @@ int glu_gfix_rng_draws(uint32_t seed, int nthreads, int thread, size_t n, double
    classes: 0 derivative, 1 FFT axis pass, 2 Polak-Ribiere sums, 3 CG
    update, 4 line search, 5 gauge update (FASD), 6 gauge update fused with
    the next derivative (FACG), 7 Lie-field packing and 8 momentum
-   projection of the gluon propagators; +9 for the FP32 variants.
+   projection of the gluon propagators; the Coulomb slice kernels 9
+   derivative, 10 Polak-Ribiere sums, 11 CG update, 12 line search,
+   13 slice-transform update; +14 for the FP32 variants.
    `mask` selects the classes timed (bit k = class k; -1 all, 0 off).
    While timing is on, a Landau run enqueues one iteration per batch, so
    no timed launch falls past the stop, and line searches skipped below
@@ -349,10 +358,11 @@ int glu_gfix_autotune(glu_gfix_ctx *ctx, char *report, size_t cap);
    worker thread, so short runs overtake long ones.  Contexts are pooled
    and reused per shape.  A request's links are a device buffer in GLU's
    layout double[V][4][18], fixed in place (Landau requests: glu_gfix_landau
-   semantics, restarts included; Coulomb: glu_gfix_run); the caller keeps it
-   alive until glu_gfix_server_wait returns.  Results are bitwise those of
-   the same call on a context of their own.  Latency runs from submit to the output copy's
-   completion. */
+   semantics, restarts included; Coulomb: glu_gfix_run, on a host thread of
+   its own so that the Landau runs in flight keep advancing); the caller
+   keeps it alive until glu_gfix_server_wait returns.  Results are bitwise
+   those of the same call on a context of their own.  Latency runs from
+   submit to the output copy's completion. */
 typedef struct glu_gfix_server glu_gfix_server;
 int glu_gfix_server_create(glu_gfix_server **srv, size_t hbm_budget,
                            int max_active);
@@ -365,6 +375,17 @@ int glu_gfix_server_submit(glu_gfix_server *srv, const size_t dims[4],
                            int gftype, int precision, int method,
                            double gf_tune, double accuracy, size_t max_iters,
                            double *links_dev, int *ticket);
+/* Submit with GLU's own input-file keys (read_gf_struct and the SEED and
+   RANDOM_TRANSFORM tags of IO/input_reader.c:188-193, 444-481, 510-517):
+   `options` holds KEY = VALUE lines (other keys are ignored).  GFTYPE
+   (LANDAU or COULOMB), ACCURACY (the exponent: 10^-ACCURACY), MAX_ITERS and
+   GF_TUNE are required.  SEED seeds the request's MWC_4096 streams (0: the
+   entropy pool, as GLU); RANDOM_TRANSFORM = YES applies GLU's
+   random_transform to the input first, and the Landau restarts then
+   continue those streams; without it they start from SEED (default: the
+   server's restart seed).  Method: FACG (CG). */
+int glu_gfix_server_submit_glu(glu_gfix_server *srv, const size_t dims[4], int precision,
+                               const char *options, double *links_dev, int *ticket);
 /* Blocks until the request ends; returns its error if it failed. */
 int glu_gfix_server_wait(glu_gfix_server *srv, int ticket, double *theta,
                          size_t *iters, double *latency_ms);

@@ -207,3 +207,56 @@ def test_luxury_copies_pick_the_smallest_functional():
         assert it == its[-1] + fin and th == th_ref
         assert np.array_equal(g.get_links(), W_ref)
         assert len(set(np.round(funcs, 12))) > 1  # the copies differ
+
+
+def test_glu_input_keys_request():
+    """A request given as GLU input-file keys: RANDOM_TRANSFORM = YES with
+    SEED = 7 transforms the input with the streams seeded 7, then fixes it
+    to 10^-ACCURACY -- bitwise the same as doing those steps alone."""
+    dims = (8, 8, 8, 16)
+    U, _ = heatbath_config(dims, sweeps=20, seed=8, gseed=108)
+    opts = "MODE = GAUGE_FIXING\nGFTYPE = LANDAU\nACCURACY = 14\nMAX_ITERS = 1000\nGF_TUNE = 0.08\n" \
+           "SEED = 7\nRANDOM_TRANSFORM = YES\n"
+    with gl.Gfix(dims, gl.LANDAU, gl.MIXED) as g:
+        g.set_links(U)
+        g.rng_seed(7, 256)
+        g.glu_random_transform()
+        it0, th0, _ = g.landau(gl.CG, 0.08, 1e-14, 1000, restart_seed=0)
+        W0 = g.get_links()
+    b = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+    torch.cuda.synchronize()
+    with gl.Server(max_active=2) as srv:
+        it, th, _ = srv.wait(srv.submit_glu(dims, b.data_ptr(), opts, gl.MIXED))
+        with pytest.raises(gl.GfixError, match="GF_TUNE"):
+            srv.submit_glu(dims, b.data_ptr(), "GFTYPE = LANDAU\nACCURACY = 14\nMAX_ITERS = 10\n")
+        with pytest.raises(gl.GfixError, match="MAG"):
+            srv.submit_glu(dims, b.data_ptr(), "GFTYPE = MAG\nACCURACY = 14\nMAX_ITERS = 10\nGF_TUNE = 0.1\n")
+    assert (it, th) == (it0, th0) and th < 1e-14
+    assert np.array_equal(b.cpu().numpy(), W0)
+
+
+def test_coulomb_request_does_not_hold_up_landau_runs():
+    """A Coulomb request runs on a host thread of its own: Landau requests
+    submitted after it finish while it is still running, with the results
+    they have alone."""
+    dc, dl = (12, 12, 12, 16), (8, 8, 8, 8)
+    Uc, _ = heatbath_config(dc, sweeps=20, seed=21, gseed=121)
+    Ul, _ = heatbath_config(dl, sweeps=20, seed=22, gseed=122)
+    with gl.Gfix(dc, gl.COULOMB, gl.FP64) as g:
+        g.set_links(Uc)
+        it, th = g.run(gl.CG, 0.09, 1e-14, 650)
+        Wc = (g.get_links(), it, th)
+    Wl = alone(dl, Ul, gl.LANDAU, gl.FP64, landau=True)
+    bc = torch.from_numpy(np.ascontiguousarray(Uc)).cuda()
+    bl = [torch.from_numpy(np.ascontiguousarray(Ul)).cuda() for _ in range(3)]
+    torch.cuda.synchronize()
+    with gl.Server(max_active=4) as srv:
+        tc = srv.submit(dc, bc.data_ptr(), gl.COULOMB, gl.FP64, gl.CG, 0.09, 1e-14, 650)
+        tl = [srv.submit(dl, b.data_ptr(), gl.LANDAU, gl.FP64, gl.CG, 0.08, 1e-14, 1000) for b in bl]
+        rl = [srv.wait(t) for t in tl]
+        rc = srv.wait(tc)
+    print(f"[serve] coulomb {rc[2]:.0f} ms; landau {[round(r[2]) for r in rl]} ms", flush=True)
+    assert max(r[2] for r in rl) < rc[2]  # the short runs overtook the long Coulomb run
+    assert (rc[0], rc[1]) == (Wc[1], Wc[2]) and np.array_equal(bc.cpu().numpy(), Wc[0])
+    for r, b in zip(rl, bl):
+        assert (r[0], r[1]) == (Wl[1], Wl[2]) and np.array_equal(b.cpu().numpy(), Wl[0])
