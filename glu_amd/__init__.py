@@ -37,6 +37,7 @@ EXPORTS = [
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
     "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional", "glu_gfix_rng_seed", "glu_gfix_glu_random_transform",
     "glu_gfix_rng_draws", "glu_gfix_phase_ms", "glu_gfix_server_submit_glu",
+    "glu_gfix_slice_trace", "glu_gfix_glu_heatbath",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
@@ -76,6 +77,7 @@ def lib():
                                                      C.POINTER(SZ), C.POINTER(C.c_int), DP, DP, SZ, DP]),
             "glu_gfix_mixed_split": (C.c_int, [P, C.POINTER(SZ), C.POINTER(SZ)]),
             "glu_gfix_slice_info": (C.c_int, [P, C.POINTER(SZ), DP]),
+            "glu_gfix_slice_trace": (C.c_int, [P, SZ, DP, SZ, C.POINTER(SZ)]),
             "glu_gfix_stage_deriv": (C.c_int, [P, DP, DP]),
             "glu_gfix_stage_fa": (C.c_int, [P, DP, DP]),
             "glu_gfix_stage_line_search": (C.c_int, [P, DP, DP, DP]),
@@ -103,6 +105,7 @@ def lib():
             "glu_gfix_server_destroy": (C.c_int, [P]),
             "glu_gfix_rng_seed": (C.c_int, [P, C.c_uint32, C.c_int]),
             "glu_gfix_glu_random_transform": (C.c_int, [P]),
+            "glu_gfix_glu_heatbath": (C.c_int, [P, C.c_double, C.c_int, C.c_int, C.c_int]),
             "glu_gfix_rng_draws": (C.c_int, [C.c_uint32, C.c_int, C.c_int, SZ, DP]),
             "glu_gfix_phase_ms": (C.c_int, [P, DP]),
             "glu_gfix_server_submit_glu": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_char_p, C.c_void_p,
@@ -270,6 +273,12 @@ class Gfix:
         _check(lib().glu_gfix_mixed_split(self._h, C.byref(a), C.byref(b)), "mixed_split")
         return int(a.value), int(b.value)
 
+    def slice_trace(self, slice_: int, cap: int = 64) -> np.ndarray:
+        rows = np.zeros((cap, 3))
+        n = SZ()
+        _check(lib().glu_gfix_slice_trace(self._h, slice_, _dp(rows), cap, C.byref(n)), "slice_trace")
+        return rows[: n.value].copy()
+
     def slice_info(self):
         T = self.dims[3]
         its = (SZ * T)()
@@ -291,6 +300,11 @@ class Gfix:
     def glu_random_transform(self):
         """GLU's random_transform drawn from the context's streams."""
         _check(lib().glu_gfix_glu_random_transform(self._h), "glu_random_transform")
+
+    def glu_heatbath(self, beta=6.0, sweeps=200, nor=4, cold_start=True):
+        """GLU's own HB + OR update (Update/KPHB.c) from the context's
+        streams (seed them with rng_seed first: SEED, Nthreads)."""
+        _check(lib().glu_gfix_glu_heatbath(self._h, beta, sweeps, nor, int(cold_start)), "glu_heatbath")
 
     def profile(self, mask=-1):
         """Time kernel classes with HIP events (bit k = KERNELS[k], bit

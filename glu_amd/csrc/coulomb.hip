@@ -27,10 +27,27 @@ struct SliceCtrl {
   double *dev;                                                   // unitarity deviation
   long long *iters;
   int *done, *did_ls;
+  // per-slice trajectory rows (theta_k, step, beta_k) of the current fix,
+  // kSliceTraceCap per slice: row 0 the SD start, then one per iteration,
+  // restarts and continuations appended
+  double *trace;
+  long long *trace_n;
   double acc, gf_alpha, cg_tol, norm;
   long long max_iters;
   int T;
 };
+
+constexpr int kSliceTraceCap = 64;
+
+__device__ __forceinline__ void slice_rec(const SliceCtrl &sc, int t, double theta, double alpha, double beta) {
+  const long long n = sc.trace_n[t];
+  if (n >= kSliceTraceCap) return;
+  double *r = sc.trace + ((long long)t * kSliceTraceCap + n) * 3;
+  r[0] = theta;
+  r[1] = alpha;
+  r[2] = beta;
+  sc.trace_n[t] = n + 1;
+}
 
 struct CoulombState {
   void *G = nullptr;      // slice gauge transforms, one per site (column planes)
@@ -291,6 +308,9 @@ __global__ void k_cfin_ls(const double *__restrict__ part, long long nbs, SliceC
       sc.alpha[t] = sc.gf_alpha;
       sc.did_ls[t] = 0;
     }
+    // FACG row: theta measured before the step, the step, beta (0 at the
+    // SD start); FASD rows are written once theta is known (k_cfin_der)
+    if (!fixed) slice_rec(sc, t, sc.theta[t], sc.alpha[t], force ? 0.0 : sc.beta[t]);
   }
 }
 
@@ -316,7 +336,10 @@ __global__ void k_cfin_der(const double *__restrict__ part, long long nbs, Slice
   double a = 0.0;
   for (long long k = threadIdx.x; k < nbs; k += blockDim.x) a += part[2 * (t * nbs + k)];
   const double s1 = block_reduce_1024(a, sh);
-  if (threadIdx.x == 0) sc.theta[t] = s1 * sc.norm;
+  if (threadIdx.x == 0) {
+    sc.theta[t] = s1 * sc.norm;
+    slice_rec(sc, t, sc.theta[t], sc.gf_alpha, 0.0);
+  }
 }
 
 __global__ void k_citer_end(SliceCtrl sc) {
@@ -390,7 +413,7 @@ int coulomb_alloc(glu_gfix_ctx *c) {
   cs->nbs = (c->g.Lcu + kBlock - 1) / kBlock;
   if (hipMalloc(&cs->G, (size_t)c->g.V * 6 * sizeof(double2)) != hipSuccess)
     return gfix_fail(-2, "slice transform allocation failed");
-  const size_t nd = 11 * (size_t)T, nl = T, ni = 2 * (size_t)T;
+  const size_t nd = (11 + 3 * (size_t)kSliceTraceCap) * (size_t)T, nl = 2 * (size_t)T, ni = 2 * (size_t)T;
   if (hipMalloc(&cs->arena, nd * sizeof(double) + nl * sizeof(long long) + ni * sizeof(int) + 256) != hipSuccess)
     return gfix_fail(-2, "slice control allocation failed");
   double *d = (double *)cs->arena;
@@ -403,7 +426,9 @@ int coulomb_alloc(glu_gfix_ctx *c) {
   s.alpha = d + 5 * T;
   s.probe = d + 6 * T;  // 3T
   s.dev = d + 9 * T;    // 2T spare
+  s.trace = d + 11 * T;  // 3 kSliceTraceCap T
   s.iters = (long long *)(d + nd);
+  s.trace_n = s.iters + T;
   s.done = (int *)(s.iters + nl);
   s.did_ls = s.done + T;
   s.T = T;
@@ -418,6 +443,24 @@ void coulomb_free(glu_gfix_ctx *c) {
   if (c->cs->arena) (void)hipFree(c->cs->arena);
   delete c->cs;
   c->cs = nullptr;
+}
+
+extern "C" int glu_gfix_slice_trace(glu_gfix_ctx *c, size_t slice, double *rows, size_t cap, size_t *n) {
+  if (!c || !rows || !n) return gfix_fail(-1, "null argument");
+  if (!c->cs) return gfix_fail(-1, "not a Coulomb context");
+  if (slice >= (size_t)c->g.L[3]) return gfix_fail(-1, "slice %zu beyond the %d time slices", slice, c->g.L[3]);
+  long long m = 0;
+  std::vector<double> h(3 * kSliceTraceCap);
+  const SliceCtrl &sc = c->cs->sc;
+  if (hipMemcpyAsync(&m, sc.trace_n + slice, sizeof(long long), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipMemcpyAsync(h.data(), sc.trace + slice * 3 * kSliceTraceCap, h.size() * sizeof(double),
+                     hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return gfix_fail(-2, "slice trace read failed");
+  const size_t k = std::min((size_t)m, cap);
+  std::copy(h.begin(), h.begin() + 3 * k, rows);
+  *n = k;
+  return 0;
 }
 
 extern "C" int glu_gfix_slice_info(glu_gfix_ctx *c, size_t *iters, double *theta) {
@@ -506,6 +549,12 @@ int coulomb_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_i
   sc.norm = 1.0 / (3.0 * (double)c->g.Lcu);  // GFNORM_COULOMB
   sc.max_iters = (long long)max_iters;
   hipLaunchKernelGGL(k_cidentity, dim3(nbl(c->g.V)), dim3(kBlock), 0, s, (double2 *)cs->G, c->g.V);
+  {
+    const std::vector<long long> zero(T, 0);
+    if (hipMemcpyAsync(sc.trace_n, zero.data(), T * sizeof(long long), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return gfix_fail(-2, "Coulomb trace reset failed");
+  }
   std::vector<int> active(T, 1), control(T, 0), sel(T, 0);
   std::vector<size_t> tot(T, 0);
   std::vector<double> th(T);

@@ -377,6 +377,111 @@ template bool launch_fft_xy<double>(void *, long long, int, const FftAxis &, con
 template bool launch_fft_xy<float>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
                                    hipStream_t, long long, int, long long, const int *);
 
+// The z and t axes of the 4D transform in one LDS residency: forward z,
+// forward t, x p^2, backward t, backward z, for PX consecutive x values of
+// one y (a "column" of z-t planes), after the forward x-y plane pass and
+// before the backward one.  Three passes over the fields per Fourier
+// acceleration instead of five.  The pencils' arithmetic is that of the
+// single-axis passes (same radix plan, twiddles and p^2 summation order).
+// Element (j, z, t) of the column sits at lds[t RZ + z PX + j]; a t row is
+// padded to RZ = NZ PX + 1 entries.  Workgroups are dealt to the 8 XCDs
+// round-robin; logical block l = (b % 8) nb/8 + b / 8 gives each XCD a
+// contiguous run of columns, so the x groups sharing a cache line are
+// fetched into one L2.
+template <typename T, int NZ, int NT, int PX>
+__global__ __launch_bounds__(kFftThreads) void k_fft_zt(vec2<T> *__restrict__ F, long long V, int Lx, int Ly,
+                                                       const double2 *__restrict__ twz,
+                                                       const double2 *__restrict__ twt, PsqInfo ps,
+                                                       const Ctrl *__restrict__ ctrl) {
+  constexpr int RZ = NZ * PX + 1, NE = NZ * NT * PX;
+  if (ctrl && ctrl->done) return;
+  __shared__ vec2<T> lds[NT * RZ];
+  __shared__ cx<T> tz[NZ], tt[NT];
+  for (int k = threadIdx.x; k < NZ; k += kFftThreads) tz[k] = mk<T>((T)twz[k].x, (T)twz[k].y);
+  for (int k = threadIdx.x; k < NT; k += kFftThreads) tt[k] = mk<T>((T)twt[k].x, (T)twt[k].y);
+  // logical block -> (field, y, x group); gridDim.x is a multiple of 8
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned l = (b & 7u) * (nb >> 3) + (b >> 3);
+  const int ngx = Lx / PX;
+  const unsigned cols = (unsigned)(ngx * Ly);
+  const int field = (int)(l / cols), col = (int)(l % cols);
+  const int xg = col % ngx, y = col / ngx;
+  const long long Sz = (long long)Lx * Ly, St = Sz * NZ;
+  vec2<T> *base = F + (long long)field * V + (long long)xg * PX + (long long)y * Lx;
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NE; e += kFftThreads) {
+    const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+    lds[t * RZ + z * PX + j] = base[j + z * Sz + t * St];
+  }
+  __syncthreads();
+  // forward z: pencil (j, t) at j + t RZ, points PX apart
+  fast_run<T, NZ, PX * NT, 0, 1, PX, 1, PX, RZ>(lds, tz, -1);
+  // forward t: pencil (j, z) at z PX + j, points RZ apart
+  fast_run<T, NT, PX * NZ, 0, 1, RZ, 1>(lds, tt, -1);
+  // p^2 = 2 (4 - sum cos) with p^2(0) := 1 (Geometry/geometry.c:170-186),
+  // the cos of x, y, z, t added in that order
+  const double *cx_ = ps.cosv + ps.off[0], *cy_ = ps.cosv + ps.off[1];
+  const double *cz_ = ps.cosv + ps.off[2], *ct_ = ps.cosv + ps.off[3];
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NE; e += kFftThreads) {
+    const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+    double kc = 0.0;
+    kc += cx_[xg * PX + j];
+    kc += cy_[y];
+    kc += cz_[z];
+    const double kcos = kc + ct_[t];
+    const double p2 = (kcos == 4.0) ? 1.0 : 2.0 * (4.0 - kcos);
+    const T f = (T)(ps.max / p2);
+    const int a = t * RZ + z * PX + j;
+    const cx<T> v = ld<T>(lds + a);
+    st<T>(lds + a, mk<T>(v.re * f, v.im * f));
+  }
+  __syncthreads();
+  fast_run<T, NT, PX * NZ, 0, 1, RZ, 1>(lds, tt, +1);
+  fast_run<T, NZ, PX * NT, 0, 1, PX, 1, PX, RZ>(lds, tz, +1);
+#pragma unroll 4
+  for (int e = threadIdx.x; e < NE; e += kFftThreads) {
+    const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+    base[j + z * Sz + t * St] = lds[t * RZ + z * PX + j];
+  }
+}
+
+// pencils of a z-t column per workgroup: 2 in FP32, 1 in FP64 (32 KiB of
+// LDS at 32 x 64 either way), or fewer where the column is small
+template <typename T, int NZ, int NT> constexpr int zt_px() {
+  constexpr int bytes = NZ * NT * (int)sizeof(vec2<T>);
+  return bytes * 2 <= 40 * 1024 ? 2 : 1;
+}
+
+template <typename T, int NZ, int NT>
+static bool try_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch) {
+  if (az.N != NZ || at.N != NT) return false;
+  constexpr int PX = zt_px<T, NZ, NT>();
+  if (ax.N % PX) return false;
+  const long long nb = (long long)(ax.N / PX) * ay.N * nfield;
+  if (nb % 8 || nb > (1ll << 30)) return false;
+  if (!launch) return true;
+  hipLaunchKernelGGL((k_fft_zt<T, NZ, NT, PX>), dim3((unsigned)nb), dim3(kFftThreads), 0, s, (vec2<T> *)F, V,
+                     ax.N, ay.N, az.tw, at.tw, ps, ctrl);
+  return true;
+}
+
+template <typename T>
+bool launch_fft_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch) {
+  if (ps.D != 4) return false;
+  return try_zt<T, 8, 8>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
+         try_zt<T, 16, 16>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
+         try_zt<T, 16, 32>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
+         try_zt<T, 24, 48>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
+         try_zt<T, 32, 64>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch);
+}
+template bool launch_fft_zt<double>(void *, long long, int, const FftAxis &, const FftAxis &, const FftAxis &,
+                                    const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool);
+template bool launch_fft_zt<float>(void *, long long, int, const FftAxis &, const FftAxis &, const FftAxis &,
+                                   const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool);
+
 template <typename T, int N, int P>
 __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ F, long long fstride, long long S,
                                                          long long npencil, const double2 *__restrict__ tw, int mode,

@@ -190,6 +190,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     CK(hipEventCreateWithFlags(&c->run_ev, hipEventDisableTiming));
     if (const char *e = getenv("GLU_GFIX_GRAPH")) c->use_graph = atoi(e);
     if (const char *e = getenv("GLU_GFIX_BATCH")) c->batch = std::max(1, std::min(64, atoi(e)));
+    if (const char *e = getenv("GLU_GFIX_ZT")) c->use_zt = atoi(e);
     CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->U64b, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
@@ -377,6 +378,13 @@ template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, co
     return ok;
   };
   const bool fused = xy(0);
+  if (fused && D == 4 && c->use_zt &&
+      launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream, false)) {
+    GFIX_PROF(c, pk<T>(PK_FFT),
+              launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream));
+    xy(1);
+    return;
+  }
   for (int a = fused ? 2 : 0; a < D - 1; a++)
     GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 0, c->ps, ctrl, c->stream, ns, sl, slice_done));
   GFIX_PROF(c, pk<T>(PK_FFT),

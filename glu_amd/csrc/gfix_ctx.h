@@ -76,6 +76,7 @@ struct glu_gfix_ctx {
   // from the lattice size
   double stall_theta = 1e-6, stall_factor = 0.0;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
+  int use_zt = 1;     // Landau: z and t axes with p^2 in one pass after the x-y pass
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner
   int ord_eg = 1, ord_ls = 1;

@@ -95,6 +95,42 @@ __device__ __forceinline__ long long nb_fwd(const Geo &g, long long i, const int
   return x[mu] < g.L[mu] - 1 ? i + g.S[mu] : i - (long long)(g.L[mu] - 1) * g.S[mu];
 }
 
+// ------------------------------------------------------------ derivative --
+// Gfix/lin_derivs.c:75-193 with the packing of Gfix/FACG.c:38-73:
+// Delta(x) = sum_mu [A_mu(x - mu) - A_mu(x)] from the own link q = U_mu(x)
+// and the backward link s = U_mu(x - mu), accumulated direction by
+// direction, then packed into the 4 fields with the local tr Delta^2
+template <typename T> struct DerivAcc {
+  T R0 = 0, R1 = 0, I1 = 0, R2 = 0, I2 = 0, R3 = 0, R4 = 0, I4 = 0;
+  __device__ __forceinline__ void add(const su3<T> &q, const su3<T> &s) {
+    R0 += T(2) * s.e[0].im - s.e[4].im - s.e[8].im;
+    R0 += T(-2) * q.e[0].im + q.e[4].im + q.e[8].im;
+    R1 += s.e[1].re - s.e[3].re - q.e[1].re + q.e[3].re;
+    I1 += s.e[1].im + s.e[3].im - q.e[1].im - q.e[3].im;
+    R2 += s.e[2].re - s.e[6].re - q.e[2].re + q.e[6].re;
+    I2 += s.e[2].im + s.e[6].im - q.e[2].im - q.e[6].im;
+    R3 += T(2) * s.e[4].im - s.e[0].im - s.e[8].im;
+    R3 += T(-2) * q.e[4].im + q.e[0].im + q.e[8].im;
+    R4 += s.e[5].re - s.e[7].re - q.e[5].re + q.e[7].re;
+    I4 += s.e[5].im + s.e[7].im - q.e[5].im - q.e[7].im;
+  }
+  // local tr Delta^2 (lin_derivs.c:180-183)
+  __device__ __forceinline__ double tr2() const {
+    const double r0 = R0, r3 = R3;
+    return 0.4444444444444444 * (r0 * r0 + r3 * (r3 + r0)) +
+           ((double)R1 * R1 + (double)I1 * I1 + (double)R2 * R2 + (double)I2 * I2 + (double)R4 * R4 +
+            (double)I4 * I4);
+  }
+  // packing of Gfix/FACG.c:57-60; returns tr Delta^2
+  __device__ __forceinline__ double store(vec2<T> *__restrict__ F, long long V, long long i) const {
+    st<T>(F + i, mk<T>(-R3 / T(3), R0 / T(3)));
+    st<T>(F + V + i, mk<T>(T(0.5) * R1, T(0.5) * I1));
+    st<T>(F + 2 * V + i, mk<T>(T(0.5) * R2, T(0.5) * I2));
+    st<T>(F + 3 * V + i, mk<T>(T(0.5) * R4, T(0.5) * I4));
+    return tr2();
+  }
+};
+
 // Device-resident scalars of one FACG/FASD run.  Every per-iteration kernel
 // reads `done` first and returns when it is set, so a captured batch of
 // iterations becomes a no-op past convergence.

@@ -44,6 +44,14 @@ bool launch_fft_xy(void *F, long long fstride, int nfield, const FftAxis &ax, co
                    const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
                    const int *slice_done);
 
+// the z and t axes of the 4D transform with p^2 in between (forward z, t,
+// x p^2, backward t, z) in one pass over the fields; false when this shape
+// has no z-t kernel (the caller runs axis passes); launch = false only
+// answers whether it has one
+template <typename T>
+bool launch_fft_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch = true);
+
 template <typename T>
 void launch_deriv(const void *U, void *F, double *part, const Geo &g, const Ctrl *ctrl, hipStream_t s);
 template <typename T>

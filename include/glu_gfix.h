@@ -269,6 +269,12 @@ int glu_gfix_mixed_split(glu_gfix_ctx *ctx, size_t *fp32_iters,
 /* Per-slice results of the last Coulomb run (T entries each). */
 int glu_gfix_slice_info(glu_gfix_ctx *ctx, size_t *iters, double *theta);
 
+/* Coulomb: trajectory of one time slice in the last run, rows
+   (theta_k, step, beta_k) as the slice's FACG steps took them
+   (Gfix/CFACG.c:286-382): row 0 the steepest-descent start, then one row per
+   iteration, continuations and restarts appended; at most 64 rows kept. */
+int glu_gfix_slice_trace(glu_gfix_ctx *ctx, size_t slice, double *rows, size_t cap, size_t *n);
+
 /* Stage-level entry points (used by the parity tests and the profiler):
    the derivative + packing of Gfix/FACG.c:38-73 (fields as 4*V complex =
    8*V doubles, field-major) with its theta; the Fourier acceleration of
@@ -311,6 +317,15 @@ int glu_gfix_rng_seed(glu_gfix_ctx *ctx, uint32_t seed, int nthreads);
    streams, then U_mu(x) <- g(x) U_mu(x) g(x + mu)^dag.  The streams must be
    seeded. */
 int glu_gfix_glu_random_transform(glu_gfix_ctx *ctx);
+/* GLU's own quenched update, hb_update of Update/KPHB.c:121-257 with
+   HEADER = UNIT (cold_start) or from the device links: `sweeps` times a
+   Kennedy-Pendleton heatbath pass (Update/hb.c:156-194) over GLU's
+   draughtboard colouring and `nor` overrelaxation passes
+   (Update/relax.c:76-114), at BETA = beta, drawing from the context's
+   copy of GLU's streams (glu_gfix_rng_seed: SEED, Nthreads) in GLU's
+   order -- stream t takes libgomp's static block of each colour's sites.
+   A heatbath pass runs one lane per stream. */
+int glu_gfix_glu_heatbath(glu_gfix_ctx *ctx, double beta, int sweeps, int nor, int cold_start);
 /* The first n doubles par_rng_dbl(thread) returns after GLU's seeding with
    (seed, nthreads): the host restatement of the streams the device copies,
    for checking them against GLU's. */

@@ -187,12 +187,16 @@ enum {
   OR_PLANT_NO_CLAMP = 5,     /* Polak-Ribiere beta not clipped at 0       */
   OR_PLANT_NO_BETA = 6,      /* beta = 0: steepest descent directions     */
   OR_PLANT_NO_STALE = 7,     /* continuation without GLU's stale red[] slots */
-  OR_PLANT_OLD_KEPT = 8      /* in_old not refreshed after an iteration   */
+  OR_PLANT_OLD_KEPT = 8,     /* in_old not refreshed after an iteration   */
+  OR_PLANT_STAPLE_NODAG = 9, /* lower staple without the dagger on U_nu(x-nu) */
+  OR_PLANT_OR_SIGN = 10,     /* overrelaxation reflection with s1 unflipped */
+  OR_PLANT_KP_NOCOS = 11     /* Kennedy-Pendleton without the cos^2 factor */
 };
 void or_set_plant(int code);
 
 /* Coulomb: record the trajectory of one slice (theta_k, step, beta) of the
-   next or_coulomb calls into *tr (NULL: off) */
+   next or_coulomb calls into *tr (NULL: off); slice = (size_t)-1 records
+   every slice, tr then pointing to one trace per time slice */
 void or_coulomb_trace(or_trace *tr, size_t slice);
 
 /* ---------------- input generation (not on the hot path) ---------------- */
@@ -200,6 +204,22 @@ void or_coulomb_trace(or_trace *tr, size_t slice);
    Kennedy-Pendleton SU(2) heatbath; own seeded generator */
 void or_heatbath(const or_lat *L, double *U, double beta, int sweeps,
                  int nor, uint64_t seed, int cold_start);
+
+/* GLU's own quenched update (Update/KPHB.c:121-257): `sweeps` times a
+   Kennedy-Pendleton heatbath pass over GLU's draughtboard colouring and
+   `nor` overrelaxation passes, from GLU's MWC_4096 streams seeded
+   SEED = seed with L->nth of them, in libgomp's static blocks; from unit
+   links when cold_start */
+void or_glu_heatbath(const or_lat *L, double *U, double beta, size_t sweeps, size_t nor, uint32_t seed,
+                     int cold_start);
+/* one overrelaxation pass (Update/relax.c:76-114) */
+void or_glu_overrelax(const or_lat *L, double *U);
+/* GLU's Wilson staple sum of link (i, mu) (Smear/staples.c:63-76) */
+void or_glu_staple(const or_lat *L, const double *U, size_t i, int mu, double *stap18);
+/* draughtboard colour of site i (Geometry/draughtboard.c:48-68) */
+int or_glu_colour(const or_lat *L, size_t i);
+/* n values a0 = 1 - d of the SU(2) heatbath at `norm` (Update/hb.c:53-96) */
+void or_glu_kp_samples(uint32_t seed, double norm, size_t n, double *a0);
 
 #ifdef __cplusplus
 }

@@ -123,7 +123,8 @@ static void sum_pr(cl_ctx *C, const double *Fin, const double *Fold)
   }
 }
 
-/* trajectory of one slice (tests): or_coulomb_trace */
+/* trajectory of one slice, or of every slice (tests): or_coulomb_trace
+   with slice = (size_t)-1 takes an array of one trace per time slice */
 static or_trace *cl_trace = NULL;
 static size_t cl_trace_slice = 0;
 void or_coulomb_trace(or_trace *tr, size_t slice)
@@ -134,7 +135,12 @@ void or_coulomb_trace(or_trace *tr, size_t slice)
 static void cl_rec(size_t t, double theta, double alpha, double beta)
 {
   or_trace *tr = cl_trace;
-  if (!tr || t != cl_trace_slice || tr->n >= tr->cap) return;
+  if (!tr) return;
+  if (cl_trace_slice == (size_t)-1)
+    tr += t;
+  else if (t != cl_trace_slice)
+    return;
+  if (tr->n >= tr->cap) return;
   if (tr->theta) tr->theta[tr->n] = theta;
   if (tr->alpha) tr->alpha[tr->n] = alpha;
   if (tr->beta) tr->beta[tr->n] = beta;

@@ -105,6 +105,11 @@ def lib():
                                                C.POINTER(SZ)]),
             "or_heatbath": (None, [LP, DP, C.c_double, C.c_int, C.c_int,
                                    C.c_uint64, C.c_int]),
+            "or_glu_heatbath": (None, [LP, DP, C.c_double, SZ, SZ, C.c_uint32, C.c_int]),
+            "or_glu_overrelax": (None, [LP, DP]),
+            "or_glu_staple": (None, [LP, DP, SZ, C.c_int, DP]),
+            "or_glu_colour": (C.c_int, [LP, SZ]),
+            "or_glu_kp_samples": (None, [C.c_uint32, C.c_double, SZ, DP]),
             "or_set_plant": (None, [C.c_int]),
             "or_coulomb_trace": (None, [C.POINTER(Trace), SZ]),
         }
@@ -205,6 +210,27 @@ class Lattice:
         lib().or_heatbath(self.ref, _dp(U), beta, sweeps, nor, seed, int(cold))
         return U
 
+    def glu_heatbath(self, beta, sweeps, nor=4, seed=1, U=None):
+        """GLU's own HB + OR update from GLU's MWC streams (self.nth of
+        them); from unit links unless U is given."""
+        cold = U is None
+        if U is None:
+            U = np.zeros((self.V, 4, 18))
+        lib().or_glu_heatbath(self.ref, _dp(U), beta, sweeps, nor, seed, int(cold))
+        return U
+
+    def glu_overrelax(self, U):
+        lib().or_glu_overrelax(self.ref, _dp(U))
+        return U
+
+    def glu_staple(self, U, i, mu):
+        S = np.zeros(18)
+        lib().or_glu_staple(self.ref, _dp(U), i, mu, _dp(S))
+        return S
+
+    def glu_colours(self):
+        return np.array([lib().or_glu_colour(self.ref, i) for i in range(self.V)])
+
     def facg(self, U, gf_alpha=0.08, acc=1e-14, max_iters=1000, trace=0,
              functional=False, sd=False):
         th = C.c_double()
@@ -229,12 +255,21 @@ class Lattice:
     def coulomb(self, U, gf_alpha=0.09, acc=1e-14, max_iters=650, facg=True,
                 seed=1, trace_slice=None, trace=0):
         """Coulomb_FA; with trace_slice set, the trajectory (theta_k, step,
-        beta_k) of that slice is returned as trace_theta/alpha/beta."""
+        beta_k) of that slice is returned as trace_theta/alpha/beta;
+        trace_slice="all" returns one such dict per slice as "traces"."""
         T = self.dims[3]
         its = (SZ * T)()
         th = np.zeros(T)
         tr, bufs = None, {}
-        if trace_slice is not None:
+        if trace_slice == "all":
+            tr = (Trace * T)()
+            bufs = [{k: np.zeros(trace) for k in ("theta", "alpha", "beta")} for _ in range(T)]
+            for t in range(T):
+                tr[t].cap, tr[t].n = trace, 0
+                for k, v in bufs[t].items():
+                    setattr(tr[t], k, _dp(v))
+            lib().or_coulomb_trace(tr, C.c_size_t(-1).value)
+        elif trace_slice is not None:
             tr = Trace()
             tr.cap, tr.n = trace, 0
             for k in ("theta", "alpha", "beta"):
@@ -248,16 +283,27 @@ class Lattice:
             if tr is not None:
                 lib().or_coulomb_trace(None, 0)
         out = {"iters": int(tot), "slice_iters": list(its), "slice_theta": th}
-        if tr is not None:
+        if trace_slice == "all":
+            out["traces"] = [{"trace_" + k: v[: tr[t].n].copy() for k, v in bufs[t].items()}
+                             for t in range(T)]
+        elif tr is not None:
             for k, v in bufs.items():
                 out["trace_" + k] = v[: tr.n].copy()
         return out
+
+
+def glu_kp_samples(seed, norm, n):
+    """n SU(2) heatbath values a0 of GLU's Kennedy-Pendleton step."""
+    a0 = np.zeros(n)
+    lib().or_glu_kp_samples(seed, norm, n, _dp(a0))
+    return a0
 
 
 class planted:
     """Context manager: run the oracle with one planted mistake
     (OR_PLANT_* of glu_oracle.h), to show that a pin catches it."""
     FR_ALWAYS, MAX_FACTOR, THETA_AFTER, EGAUGE_SIGN, NO_CLAMP, NO_BETA, NO_STALE, OLD_KEPT = range(1, 9)
+    STAPLE_NODAG, OR_SIGN, KP_NOCOS = range(9, 12)
 
     def __init__(self, code: int):
         self.code = code

@@ -13,13 +13,15 @@ the three spatial directions < 1e-13).
 
 Oracle check on a cut: slices are independent problems in their spatial
 links (Gfix/CFACG.c:593-614), so the first two time slices, cut out as a
-48^3x2 lattice, must be fixed the same way by the oracle as inside the
-full batched run: per-slice iteration counts equal to within 2 (or the
-oracle's own spread between two thread counts), the gauge-invariant
-per-link traces of the spatial links to 1e-10 (or 20x that spread).  The
-temporal links of slice 0 pair slices 0 and 1 in both lattices; each slice's
-transform is fixed only up to a constant rotation, so their mean trace is
-compared to 1e-8 relative, as for the smaller lattices.
+48^3x2 lattice, start the same trajectories under the oracle as inside the
+full batched run.  Their first 21 rows (theta_k measured on the accelerated
+field, the step, beta_k) are held to 1e-12 relative, or 20x the oracle's own
+spread between two thread counts where that is larger.  The complete solves
+are not compared link by link: at this size the late trajectory is decided
+by rounding (two oracle thread counts took 402 / 438 and 1098 / 666
+iterations on these two slices and ended on different Gribov copies), so a
+slice's final transform is a property of its summation order, not of the
+algorithm.
 """
 import os
 
@@ -28,7 +30,7 @@ import pytest
 
 from oracle import oracle as O
 
-from gpu_common import rel
+from gpu_common import check_trajectory
 
 pytestmark = pytest.mark.gpu
 
@@ -49,16 +51,17 @@ def run():
         print("[config4] batched Coulomb FACG, 96 slices", flush=True)
         it, th = g.run(gl.CG, 0.09, 1e-14, 650)
         its, ths = g.slice_info()
+        rows = [g.slice_trace(t) for t in (0, 1)]
         g.reunit()
         m = g.measure()
         W = g.get_links()
     print(f"[config4] {it} slice iterations in all (per slice {min(its)}..{max(its)}), largest slice theta "
           f"{max(ths):.3e}, plaquette drift {abs(m['plaquette'] - plaq):.1e}", flush=True)
-    return U, plaq, it, th, its, ths, m, W
+    return U, plaq, it, th, its, ths, m, W, rows
 
 
 def test_every_slice_converges_and_the_plaquette_is_kept(run):
-    U, plaq, it, th, its, ths, m, W = run
+    U, plaq, it, th, its, ths, m, W, _ = run
     assert len(its) == DIMS[3] and it == sum(its)
     assert max(ths) < 1e-14 and th == max(ths)
     assert min(its) < max(its)  # uneven slices in one batched run
@@ -73,29 +76,16 @@ def test_coulomb_condition_on_the_whole_lattice(run):
 
 
 def test_two_slices_cut_out_match_the_oracle(run):
-    U, plaq, it, th, its, ths, m, W = run
+    U, plaq, it, th, its, ths, m, W, rows = run
     Lcu = DIMS[0] * DIMS[1] * DIMS[2]
     cut = (DIMS[0], DIMS[1], DIMS[2], 2)
     Uc = np.ascontiguousarray(U[: 2 * Lcu])
-    res = {}
+    traces = {}
     for nth in (NTH, max(1, NTH // 2)):
-        print(f"[config4] oracle on the 48^3x2 cut, {nth} threads", flush=True)
-        L = O.Lattice(cut, nth=nth)
-        Wc = Uc.copy()
-        r = L.coulomb(Wc, 0.09, 1e-14, 650, facg=True)
-        L.reunit(Wc)
-        res[nth] = (r, Wc)
-    (r1, W1), (r2, W2) = res[NTH], res[max(1, NTH // 2)]
-    print(f"[config4] slices 0, 1: gpu iterations {its[:2]}, oracle {r1['slice_iters']} / {r2['slice_iters']}",
-          flush=True)
-    assert max(r1["slice_theta"]) < 1e-14
-    dit = max(abs(a - b) for a, b in zip(r1["slice_iters"], r2["slice_iters"]))
-    assert max(abs(a - b) for a, b in zip(its[:2], r1["slice_iters"])) <= max(2, dit)
-    tr = lambda X: X[:, :3, 0] + X[:, :3, 8] + X[:, :3, 16]
-    Wg = W[: 2 * Lcu]
-    spread = np.abs(tr(W2) - tr(W1)).max()
-    err = np.abs(tr(Wg) - tr(W1)).max()
-    print(f"[config4] spatial per-link traces: max abs diff {err:.2e} (oracle spread {spread:.2e})", flush=True)
-    assert err < max(1e-10, 20 * spread)
-    t0 = lambda X: (X[:Lcu, 3, 0] + X[:Lcu, 3, 8] + X[:Lcu, 3, 16]).mean()
-    assert rel(t0(Wg), t0(W1)) < 1e-8
+        print(f"[config4] oracle on the 48^3x2 cut, {nth} threads, 20 iterations per slice", flush=True)
+        r = O.Lattice(cut, nth=nth).coulomb(Uc.copy(), 0.09, 1e-14, 20, facg=True, trace_slice="all", trace=21)
+        traces[nth] = r["traces"]
+    print(f"[config4] gpu full solve: slices 0, 1 took {its[:2]} iterations", flush=True)
+    for t in (0, 1):
+        assert len(rows[t]) >= 21
+        check_trajectory(f"config4 slice {t}", rows[t], traces[NTH][t], traces[max(1, NTH // 2)][t], n=21)

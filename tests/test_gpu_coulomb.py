@@ -17,7 +17,7 @@ import pytest
 from oracle import oracle as O
 
 from conftest import GOLDEN
-from gpu_common import heatbath_config, rel
+from gpu_common import check_trajectory, heatbath_config, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -78,6 +78,30 @@ def test_heatbath_batched_slices():
     sp_o, _ = links_sp_t(W)
     sp_g, _ = links_sp_t(Wg)
     assert rel(sp_g, sp_o) < 1e-10
+
+
+def test_slice_trajectories_match_the_oracle():
+    """Every slice's first 21 rows (theta_k on the accelerated field, step,
+    beta_k) against the oracle, which fixes the slices one after another:
+    1e-12 relative, or 20x the oracle's own spread over four other thread
+    counts (summation orders).  On 8^3 slices the parabola fit of the line
+    search amplifies rounding by about 10^3 over 20 iterations, so the
+    later rows of some slices carry 1e-10 differences between any two
+    summation orders, the oracle's included."""
+    dims = (8, 8, 8, 12)
+    U, _ = heatbath_config(dims, sweeps=30)
+    tr = {}
+    for nth in (4, 1, 2, 3, 8):
+        W = U.copy()
+        tr[nth] = O.Lattice(dims, nth=nth).coulomb(W, 0.09, 1e-14, 650, facg=True, trace_slice="all",
+                                                   trace=21)["traces"]
+    with gl.Gfix(dims, gl.COULOMB) as g:
+        g.set_links(U)
+        g.run(gl.CG, 0.09, 1e-14, 650)
+        rows = [g.slice_trace(t) for t in range(dims[3])]
+    for t in range(dims[3]):
+        n = min(21, len(tr[4][t]["trace_theta"]))
+        check_trajectory(f"coulomb slice {t}", rows[t], tr[4][t], [tr[k][t] for k in (1, 2, 3, 8)], n=n)
 
 
 def test_odd_spatial_extent_and_fasd():

@@ -76,7 +76,9 @@ def test_stage_derivative(cfg):
 
 @pytest.mark.parametrize("dims", [(8, 8, 8, 8), (6, 4, 12, 8), (4, 6, 8, 24), (16, 8, 4, 6), (5, 7, 10, 14),
                                   (32, 4, 2, 64), (96, 2, 48, 2), (16, 16, 4, 6), (24, 24, 2, 4),
-                                  (32, 32, 2, 3), (48, 48, 2, 2)])
+                                  (32, 32, 2, 3), (48, 48, 2, 2),
+                                  # the z-t column pass after the x-y plane pass
+                                  (16, 16, 16, 16), (16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64)])
 def test_stage_fourier_accelerate(dims):
     rng = np.random.default_rng(5)
     V = int(np.prod(dims))

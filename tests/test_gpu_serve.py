@@ -121,9 +121,11 @@ def test_admission_respects_budget_and_rejects_oversize():
 
 def test_landau_restart_policy():
     """glu_gfix_landau against the policy of Gfix/Landau.c:301-345 replayed
-    by hand: attempt 0 on the input, attempt k on the input after the
-    random transform seeded restart_seed + k; the first success ends it and
-    the count is 123456789 + (k - 1) max_iters + its iterations."""
+    by hand: attempt 0 on the input, attempt k on the input after the k-th
+    random_transform drawn from GLU's streams seeded SEED = restart_seed
+    (256 streams, seeded once before the first restart); the first success
+    ends it and the count is 123456789 + (k - 1) max_iters + its
+    iterations."""
     dims = (8, 8, 8, 8)
     U, _ = heatbath_config(dims, sweeps=20)
     seed = 5
@@ -138,9 +140,10 @@ def test_landau_restart_policy():
             if it0 != 123456789:
                 continue
             outcome = None  # (k, loc, theta) of the first successful restart
+            g.rng_seed(seed, 256)
             for k in range(1, 7):
                 g.restore()
-                g.random_transform(seed + k)
+                g.glu_random_transform()
                 loc, th = g.run(gl.CG, 0.08, 1e-14, M)
                 if loc != 123456789:
                     outcome = (k, loc, th, g.get_links())
@@ -178,8 +181,9 @@ def test_server_reports_failure_and_keeps_serving():
 
 def test_luxury_copies_pick_the_smallest_functional():
     """glu_gfix_luxury against luxury_copy_fast (Gfix/Landau.c:88-213,
-    BEST_COPY) replayed by hand: copy k is the input under the transform
-    seeded seed + k, fixed to 1e-6 in at most 1000 iterations; the smallest
+    BEST_COPY) replayed by hand: copy k is the input under the k-th
+    random_transform drawn from GLU's streams seeded SEED = seed (256
+    streams), fixed to 1e-6 in at most 1000 iterations; the smallest
     functional among converged copies is then fixed to the accuracy."""
     dims = (8, 8, 8, 8)
     U, _ = heatbath_config(dims, sweeps=20)
@@ -187,18 +191,18 @@ def test_luxury_copies_pick_the_smallest_functional():
     with gl.Gfix(dims, gl.LANDAU, gl.FP64) as g:
         g.set_links(U)
         g.snapshot()
-        funcs, its = [], []
+        funcs, its, fixed = [], [], []
+        g.rng_seed(seed, 256)
         for k in range(copies):
             g.restore()
-            g.random_transform(seed + k)
+            g.glu_random_transform()
             it, _ = g.run(gl.CG, 0.08, 1e-6, 1000)
             funcs.append(g.measure()["functional"])
             its.append(it)
+            fixed.append(g.get_links())
         ok = [k for k in range(copies) if its[k] not in (1000, 123456789)]
         best = min(ok, key=lambda k: funcs[k])
-        g.restore()
-        g.random_transform(seed + best)
-        g.run(gl.CG, 0.08, 1e-6, 1000)
+        g.set_links(fixed[best])
         fin, th_ref = g.run(gl.CG, 0.08, 1e-14, 1000)
         W_ref = g.get_links()
         g.restore()
