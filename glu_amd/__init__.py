@@ -36,7 +36,7 @@ EXPORTS = [
     "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
     "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional", "glu_gfix_rng_seed", "glu_gfix_glu_random_transform",
-    "glu_gfix_rng_draws", "glu_gfix_phase_ms", "glu_gfix_server_submit_glu",
+    "glu_gfix_rng_draws", "glu_gfix_phase_ms", "glu_gfix_server_submit_glu", "glu_gfix_measure_noalloc",
     "glu_gfix_slice_trace", "glu_gfix_glu_heatbath",
 ]
 
@@ -110,6 +110,7 @@ def lib():
             "glu_gfix_phase_ms": (C.c_int, [P, DP]),
             "glu_gfix_server_submit_glu": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_char_p, C.c_void_p,
                                                      C.POINTER(C.c_int)]),
+            "glu_gfix_measure_noalloc": (C.c_int, [C.c_void_p, C.POINTER(SZ), C.c_void_p, C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

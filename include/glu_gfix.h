@@ -194,6 +194,21 @@ int glu_gfix_theta(glu_gfix_ctx *ctx, double *theta_lin);
 /* Reunitarise every device link (Geometry/random_config.c:43-56). */
 int glu_gfix_reunit(glu_gfix_ctx *ctx);
 
+/* Allocation-free variant of glu_gfix_measure + glu_gfix_theta.  It has no
+   counterpart in GLU; it is for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops).  It reads caller-owned
+   device links in GLU's layout (double[V][4][18], x fastest, as
+   glu_gfix_set_links_device takes them) and writes 6 doubles to the device
+   buffer out_dev, enqueued on `stream` (a hipStream_t; NULL: the default
+   stream) without waiting: plaquette, link trace, largest link trace,
+   functional, temporal link trace, theta_test_lin.  Extents 2..4096 and
+   fewer than 2^31 sites; larger lattices are refused (-1).  Scratch: a fixed
+   64 KiB device array per GPU, held until the process ends.  Calls that
+   overlap on one GPU share it and race: serialise them.  Call it once
+   outside graph capture on each device first (that call looks the array up). */
+int glu_gfix_measure_noalloc(const double *links_dev, const size_t dims[4],
+                             double *out_dev, void *stream);
+
 /* GLU's momentum-space cuts of gauge-fixed links, replacing
    Cuts/cuts.c:114-184 (prototype Headers/cuts.h:41) for dir = GLUON_PROPS
    and EXCEPTIONAL (GLU's default) with the LINEAR definition: the file
