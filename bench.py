@@ -376,8 +376,8 @@ def coulomb_main(a):
     Lcu = V // dims[3]
     slaunch = sum(psits) + dims[3]
     kernels = {}
-    for (k, p), (n, m, tt) in sorted(prof.items()):
-        e = {"launches": n, "mean_ms": round(m, 4), "total_ms": round(tt, 2)}
+    for (k, p), (n, mean, tt) in sorted(prof.items()):
+        e = {"launches": n, "mean_ms": round(mean, 4), "total_ms": round(tt, 2)}
         if k in ("c_deriv", "c_pr_sums", "c_cg_update", "c_slice_update"):
             cred, stored = COULOMB_KERNEL_BYTES[k]
             e["GBps"] = round(cred * Lcu * slaunch / (tt * 1e-3) / 1e9, 1)

@@ -32,6 +32,9 @@ struct SliceCtrl {
   // restarts and continuations appended
   double *trace;
   long long *trace_n;
+  // row -> time slice of the launches of a batch: only the slices still
+  // running at the batch's start get blocks
+  int *map;
   double acc, gf_alpha, cg_tol, norm;
   long long max_iters;
   int T;
@@ -73,16 +76,18 @@ __device__ __forceinline__ long long site_of(const Geo &g, long long t, long lon
 // share of every slice (slices stop at different iterations; whole slices
 // per XCD would leave some XCDs idle).  Partials are indexed by (slice,
 // block), so results do not depend on the order.
-__device__ __forceinline__ void slice_block(const Geo &g, int &t, long long &bx) {
+__device__ __forceinline__ void slice_block(const Geo &g, const SliceCtrl &sc, int &t, long long &bx) {
   const unsigned l = blockIdx.x + blockIdx.y * gridDim.x, nbs = gridDim.x;
+  int row;
   if (g.xcd && (nbs & 7u) == 0) {
     const unsigned e = nbs >> 3, x = l & 7u, k = l >> 3;
-    t = (int)(k / e);
+    row = (int)(k / e);
     bx = (long long)(x * e + k % e);
-    return;
+  } else {
+    row = (int)(l / nbs);
+    bx = (long long)(l % nbs);
   }
-  t = (int)(l / nbs);
-  bx = (long long)(l % nbs);
+  t = sc.map[row];
 }
 
 __device__ __forceinline__ void herm_short(const su3<double> &U, double s[10]) {
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_cderiv(const double2 *__restrict__ U
                                                   double2 *__restrict__ F, Geo g, SliceCtrl sc) {
   int t;
   long long bx;
-  slice_block(g, t, bx);
+  slice_block(g, sc, t, bx);
   if (sc.done[t]) return;
   const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
@@ -143,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_cpr(const double2 *__restrict__ F, c
                                                double *__restrict__ part, Geo g, SliceCtrl sc, long long nbs) {
   int t;
   long long bx;
-  slice_block(g, t, bx);
+  slice_block(g, sc, t, bx);
   if (sc.done[t]) return;
   const long long xl = bx * blockDim.x + threadIdx.x;
   double v[2] = {0.0, 0.0};
@@ -212,7 +217,7 @@ __global__ __launch_bounds__(kBlock) void k_cupdate(const double2 *__restrict__ 
                                                    SliceCtrl sc, int mode) {
   int t;
   long long bx;
-  slice_block(g, t, bx);
+  slice_block(g, sc, t, bx);
   if (sc.done[t]) return;
   const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
@@ -242,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_cline(const double2 *__restrict__ U,
                                                  SliceCtrl sc, long long nbs, int force) {
   int t;
   long long bx;
-  slice_block(g, t, bx);
+  slice_block(g, sc, t, bx);
   if (sc.done[t]) return;
   if (!force && !(sc.theta[t] > sc.cg_tol)) return;
   const long long xl = bx * blockDim.x + threadIdx.x;
@@ -319,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_cexp(double2 *__restrict__ G, const 
                                                 SliceCtrl sc) {
   int t;
   long long bx;
-  slice_block(g, t, bx);
+  slice_block(g, sc, t, bx);
   if (sc.done[t]) return;
   const long long xl = bx * blockDim.x + threadIdx.x;
   if (xl >= g.Lcu) return;
@@ -413,7 +418,7 @@ int coulomb_alloc(glu_gfix_ctx *c) {
   cs->nbs = (c->g.Lcu + kBlock - 1) / kBlock;
   if (hipMalloc(&cs->G, (size_t)c->g.V * 6 * sizeof(double2)) != hipSuccess)
     return gfix_fail(-2, "slice transform allocation failed");
-  const size_t nd = (11 + 3 * (size_t)kSliceTraceCap) * (size_t)T, nl = 2 * (size_t)T, ni = 2 * (size_t)T;
+  const size_t nd = (11 + 3 * (size_t)kSliceTraceCap) * (size_t)T, nl = 2 * (size_t)T, ni = 3 * (size_t)T;
   if (hipMalloc(&cs->arena, nd * sizeof(double) + nl * sizeof(long long) + ni * sizeof(int) + 256) != hipSuccess)
     return gfix_fail(-2, "slice control allocation failed");
   double *d = (double *)cs->arena;
@@ -431,6 +436,7 @@ int coulomb_alloc(glu_gfix_ctx *c) {
   s.trace_n = s.iters + T;
   s.done = (int *)(s.iters + nl);
   s.did_ls = s.done + T;
+  s.map = s.did_ls + T;
   s.T = T;
   cs->slice_iters.assign(T, 0);
   cs->slice_theta.assign(T, 0.0);
@@ -481,14 +487,30 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
   hipStream_t s = c->stream;
   Geo g = c->g;
   g.xcd = c->ord_cl ? 1 : 0;
-  const dim3 grid((unsigned)cs->nbs, T), blk(kBlock);
-  std::vector<int> done(T);
+  std::vector<int> done(T), rows;
   std::vector<long long> zero(T, 0);
   for (int t = 0; t < T; t++) done[t] = active[t] ? 0 : 1;
+  // the launches of a batch get blocks for the slices running at its start
+  // only (rows -> slices through sc.map); slices that stop inside a batch
+  // return at once until the next one drops them
+  auto compact = [&]() {
+    rows.clear();
+    for (int t = 0; t < T; t++)
+      if (!done[t]) rows.push_back(t);
+    if (rows.empty()) return 0;
+    if (hipMemcpyAsync(sc.map, rows.data(), rows.size() * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return gfix_fail(-2, "Coulomb slice map upload failed");
+    return 0;
+  };
   if (hipMemcpyAsync(sc.done, done.data(), T * sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipMemcpyAsync(sc.iters, zero.data(), T * sizeof(long long), hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess)
     return gfix_fail(-2, "Coulomb slice state upload failed");
+  if (int rc = compact()) return rc;
+  if (rows.empty()) return 0;
+  dim3 grid((unsigned)cs->nbs, (unsigned)rows.size());
+  const dim3 blk(kBlock);
   double2 *U = (double2 *)c->U64, *G = (double2 *)cs->G, *F = (double2 *)c->F, *sn = (double2 *)c->sn,
           *O = (double2 *)c->old;
   double *part = c->part;
@@ -498,7 +520,7 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     // state lives within this call (every call starts with an SD step), so
     // a slice that stopped may keep its fields in either buffer.
     GFIX_PROF(c, PK_CDERIV, hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc));
-    enq_fa<double>(c, F, nullptr, sc.done);
+    enq_fa<double>(c, F, nullptr, sc.done, sc.map, (int)rows.size());
     GFIX_PROF(c, PK_CPR, hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, F, part, g, sc, cs->nbs));
     hipLaunchKernelGGL(k_cfin_pr, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 0);
     GFIX_PROF(c, PK_CLINE, hipLaunchKernelGGL(k_cline, grid, blk, 0, s, U, G, F, part, g, sc, cs->nbs, 1));
@@ -510,7 +532,7 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
   for (;;) {
     for (int b = 0; b < 8; b++) {
       GFIX_PROF(c, PK_CDERIV, hipLaunchKernelGGL(k_cderiv, grid, blk, 0, s, U, G, F, g, sc));
-      enq_fa<double>(c, F, nullptr, sc.done);
+      enq_fa<double>(c, F, nullptr, sc.done, sc.map, (int)rows.size());
       if (method == GLU_GFIX_CG) {
         GFIX_PROF(c, PK_CPR, hipLaunchKernelGGL(k_cpr, grid, blk, 0, s, F, O, part, g, sc, cs->nbs));
         hipLaunchKernelGGL(k_cfin_pr, dim3(T), dim3(1024), 0, s, part, cs->nbs, sc, 1);
@@ -534,6 +556,8 @@ static int run_slices(glu_gfix_ctx *c, int method, const std::vector<int> &activ
     bool all = true;
     for (int t = 0; t < T; t++) all = all && done[t];
     if (all) break;
+    if (int rc = compact()) return rc;
+    grid.y = (unsigned)rows.size();
   }
   return 0;
 }

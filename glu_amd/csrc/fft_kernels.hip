@@ -154,12 +154,15 @@ __device__ __forceinline__ void fft_run(vec2<T> *lds, int ldP, const FftAxis &ax
 template <typename T>
 __global__ __launch_bounds__(kFftThreads) void k_fft_axis(vec2<T> *__restrict__ F, long long fstride, FftAxis ax,
                                                          int mode, PsqInfo ps, const Ctrl *__restrict__ ctrl,
-                                                         long long slice_len, const int *__restrict__ slice_done) {
+                                                         long long slice_len, const int *__restrict__ slice_done,
+    const int *__restrict__ slice_map) {
   if (ctrl && ctrl->done) return;
-  if (slice_done && slice_done[blockIdx.z]) return;
+  // batch row blockIdx.z is time slice zs (the rows of running slices only)
+  const int zs = slice_map ? slice_map[blockIdx.z] : (int)blockIdx.z;
+  if (slice_done && slice_done[zs]) return;
   extern __shared__ unsigned char fft_smem[];
   vec2<T> *lds = reinterpret_cast<vec2<T> *>(fft_smem);
-  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len;
+  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)zs * slice_len;
   const long long p0 = (long long)blockIdx.x * ax.P;
   const int ldP = ax.P + 1;
   const int tot = ax.N * ax.P;
@@ -322,16 +325,19 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_xy(vec2<T> *__restrict__ F,
                                                        const double2 *__restrict__ twx,
                                                        const double2 *__restrict__ twy, int mode,
                                                        const Ctrl *__restrict__ ctrl, long long slice_len,
-                                                       const int *__restrict__ slice_done) {
+                                                       const int *__restrict__ slice_done,
+    const int *__restrict__ slice_map) {
   constexpr int LD = NPL * NY + 1, NE = NX * NY, NT = NPL * NE;
   if (ctrl && ctrl->done) return;
-  if (slice_done && slice_done[blockIdx.z]) return;
+  // batch row blockIdx.z is time slice zs (the rows of running slices only)
+  const int zs = slice_map ? slice_map[blockIdx.z] : (int)blockIdx.z;
+  if (slice_done && slice_done[zs]) return;
   __shared__ vec2<T> lds[NX * LD];
   __shared__ cx<T> tx[NX], ty[NY];
   for (int k = threadIdx.x; k < NX; k += kFftThreads) tx[k] = mk<T>((T)twx[k].x, (T)twx[k].y);
   for (int k = threadIdx.x; k < NY; k += kFftThreads) ty[k] = mk<T>((T)twy[k].x, (T)twy[k].y);
   vec2<T> *base =
-      F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len + (long long)blockIdx.x * NT;
+      F + (long long)blockIdx.y * fstride + (long long)zs * slice_len + (long long)blockIdx.x * NT;
   // element e of the tile: plane e / NE, row (e % NE) / NX, column e % NX
 #pragma unroll 4
   for (int e = threadIdx.x; e < NT; e += kFftThreads) lds[(e % NX) * LD + e / NX] = base[e];
@@ -346,7 +352,7 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_xy(vec2<T> *__restrict__ F,
 template <typename T, int N>
 static bool try_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
                    const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
-                   const int *slice_done) {
+                   const int *slice_done, const int *slice_map) {
   if (ax.N != N || ay.N != N) return false;
   if (sizeof(T) * 2 * N * (N + 1) > 48 * 1024) return false;
   // two planes per tile in FP32 where a plane has at most 1024 points (the
@@ -355,27 +361,27 @@ static bool try_xy(void *F, long long fstride, int nfield, const FftAxis &ax, co
   constexpr int NPL = (sizeof(T) == 4 && N * N <= 1024) ? 2 : 1;
   if (NPL == 2 && nplane % 2 == 0)
     hipLaunchKernelGGL((k_fft_xy<T, N, N, NPL>), dim3((unsigned)(nplane / 2), nfield, nslice), dim3(kFftThreads),
-                       0, s, (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
+                       0, s, (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done, slice_map);
   else
     hipLaunchKernelGGL((k_fft_xy<T, N, N, 1>), dim3((unsigned)nplane, nfield, nslice), dim3(kFftThreads), 0, s,
-                       (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done);
+                       (vec2<T> *)F, fstride, ax.tw, ay.tw, mode, ctrl, slice_len, slice_done, slice_map);
   return true;
 }
 
 template <typename T>
 bool launch_fft_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
                    const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
-                   const int *slice_done) {
-  return try_xy<T, 8>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
-         try_xy<T, 16>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
-         try_xy<T, 24>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
-         try_xy<T, 32>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done) ||
-         try_xy<T, 48>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done);
+                   const int *slice_done, const int *slice_map) {
+  return try_xy<T, 8>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done, slice_map) ||
+         try_xy<T, 16>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done, slice_map) ||
+         try_xy<T, 24>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done, slice_map) ||
+         try_xy<T, 32>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done, slice_map) ||
+         try_xy<T, 48>(F, fstride, nfield, ax, ay, mode, ctrl, s, nplane, nslice, slice_len, slice_done, slice_map);
 }
 template bool launch_fft_xy<double>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
-                                    hipStream_t, long long, int, long long, const int *);
+                                    hipStream_t, long long, int, long long, const int *, const int *);
 template bool launch_fft_xy<float>(void *, long long, int, const FftAxis &, const FftAxis &, int, const Ctrl *,
-                                   hipStream_t, long long, int, long long, const int *);
+                                   hipStream_t, long long, int, long long, const int *, const int *);
 
 // The z and t axes of the 4D transform in one LDS residency: forward z,
 // forward t, x p^2, backward t, backward z, for PX consecutive x values of
@@ -408,10 +414,29 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_zt(vec2<T> *__restrict__ F,
   const int xg = col % ngx, y = col / ngx;
   const long long Sz = (long long)Lx * Ly, St = Sz * NZ;
   vec2<T> *base = F + (long long)field * V + (long long)xg * PX + (long long)y * Lx;
-#pragma unroll 4
-  for (int e = threadIdx.x; e < NE; e += kFftThreads) {
-    const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
-    lds[t * RZ + z * PX + j] = base[j + z * Sz + t * St];
+  // the column's loads are all issued before any is stored to LDS: with
+  // runs of only PX elements per (z, t) row the pass is bound by the bytes
+  // each thread keeps in flight
+  if constexpr (NE % kFftThreads == 0) {
+    constexpr int K = NE / kFftThreads;
+    vec2<T> r[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = threadIdx.x + k * kFftThreads;
+      const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+      r[k] = base[j + z * Sz + t * St];
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      const int e = threadIdx.x + k * kFftThreads;
+      const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+      lds[t * RZ + z * PX + j] = r[k];
+    }
+  } else {
+    for (int e = threadIdx.x; e < NE; e += kFftThreads) {
+      const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
+      lds[t * RZ + z * PX + j] = base[j + z * Sz + t * St];
+    }
   }
   __syncthreads();
   // forward z: pencil (j, t) at j + t RZ, points PX apart
@@ -439,25 +464,29 @@ __global__ __launch_bounds__(kFftThreads) void k_fft_zt(vec2<T> *__restrict__ F,
   __syncthreads();
   fast_run<T, NT, PX * NZ, 0, 1, RZ, 1>(lds, tt, +1);
   fast_run<T, NZ, PX * NT, 0, 1, PX, 1, PX, RZ>(lds, tz, +1);
-#pragma unroll 4
+#pragma unroll 8
   for (int e = threadIdx.x; e < NE; e += kFftThreads) {
     const int j = e % PX, z = (e / PX) % NZ, t = e / (PX * NZ);
     base[j + z * Sz + t * St] = lds[t * RZ + z * PX + j];
   }
 }
 
-// pencils of a z-t column per workgroup: 2 in FP32, 1 in FP64 (32 KiB of
-// LDS at 32 x 64 either way), or fewer where the column is small
-template <typename T, int NZ, int NT> constexpr int zt_px() {
+// x values (pencils) of a z-t column per workgroup: variant 0 the largest
+// power of two whose tile fits 40 KiB of LDS (2 in FP32, 1 in FP64 at
+// 32 x 64), variant 1 half of it, variant 2 twice it where the tile stays
+// within 96 KiB (gfx950 gives a workgroup up to 160 KiB)
+template <typename T, int NZ, int NT> constexpr int zt_px(int v) {
   constexpr int bytes = NZ * NT * (int)sizeof(vec2<T>);
-  return bytes * 2 <= 40 * 1024 ? 2 : 1;
+  int p = 1;
+  while (p * 2 * bytes <= 40 * 1024 && p < 8) p *= 2;
+  if (v == 1) return p > 1 ? p / 2 : p;
+  if (v == 2) return (2 * p * NZ + 1) * NT * (int)sizeof(vec2<T>) <= 98304 ? 2 * p : p;
+  return p;
 }
 
-template <typename T, int NZ, int NT>
-static bool try_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
-                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch) {
-  if (az.N != NZ || at.N != NT) return false;
-  constexpr int PX = zt_px<T, NZ, NT>();
+template <typename T, int NZ, int NT, int PX>
+static bool zt_go(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
+                  const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch) {
   if (ax.N % PX) return false;
   const long long nb = (long long)(ax.N / PX) * ay.N * nfield;
   if (nb % 8 || nb > (1ll << 30)) return false;
@@ -467,35 +496,47 @@ static bool try_zt(void *F, long long V, int nfield, const FftAxis &ax, const Ff
   return true;
 }
 
+template <typename T, int NZ, int NT>
+static bool try_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch, int v) {
+  if (az.N != NZ || at.N != NT) return false;
+  if (v == 1) return zt_go<T, NZ, NT, zt_px<T, NZ, NT>(1)>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch);
+  if (v == 2) return zt_go<T, NZ, NT, zt_px<T, NZ, NT>(2)>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch);
+  return zt_go<T, NZ, NT, zt_px<T, NZ, NT>(0)>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch);
+}
+
 template <typename T>
 bool launch_fft_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
-                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch) {
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch, int v) {
   if (ps.D != 4) return false;
-  return try_zt<T, 8, 8>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
-         try_zt<T, 16, 16>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
-         try_zt<T, 16, 32>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
-         try_zt<T, 24, 48>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch) ||
-         try_zt<T, 32, 64>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch);
+  return try_zt<T, 8, 8>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch, v) ||
+         try_zt<T, 16, 16>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch, v) ||
+         try_zt<T, 16, 32>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch, v) ||
+         try_zt<T, 24, 48>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch, v) ||
+         try_zt<T, 32, 64>(F, V, nfield, ax, ay, az, at, ps, ctrl, s, launch, v);
 }
 template bool launch_fft_zt<double>(void *, long long, int, const FftAxis &, const FftAxis &, const FftAxis &,
-                                    const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool);
+                                    const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool, int);
 template bool launch_fft_zt<float>(void *, long long, int, const FftAxis &, const FftAxis &, const FftAxis &,
-                                   const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool);
+                                   const FftAxis &, const PsqInfo &, const Ctrl *, hipStream_t, bool, int);
 
 template <typename T, int N, int P>
 __global__ __launch_bounds__(kFftThreads) void k_fft_fast(vec2<T> *__restrict__ F, long long fstride, long long S,
                                                          long long npencil, const double2 *__restrict__ tw, int mode,
                                                          PsqInfo ps, const Ctrl *__restrict__ ctrl,
-                                                         long long slice_len, const int *__restrict__ slice_done) {
+                                                         long long slice_len, const int *__restrict__ slice_done,
+    const int *__restrict__ slice_map) {
   constexpr int LD = P + 1, TOT = N * P;
   static_assert(LD > 0, "");
   static_assert(kFftThreads % P == 0, "a pencil per thread needs P | threads");
   if (ctrl && ctrl->done) return;
-  if (slice_done && slice_done[blockIdx.z]) return;
+  // batch row blockIdx.z is time slice zs (the rows of running slices only)
+  const int zs = slice_map ? slice_map[blockIdx.z] : (int)blockIdx.z;
+  if (slice_done && slice_done[zs]) return;
   __shared__ vec2<T> lds[N * LD];
   __shared__ cx<T> twl[N];
   for (int k = threadIdx.x; k < N; k += kFftThreads) twl[k] = mk<T>((T)tw[k].x, (T)tw[k].y);
-  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)blockIdx.z * slice_len;
+  vec2<T> *base = F + (long long)blockIdx.y * fstride + (long long)zs * slice_len;
   const long long p0 = (long long)blockIdx.x * P;
   // this thread's pencil and first element (strided axes)
   const int jt = threadIdx.x % P, n0 = threadIdx.x / P;
@@ -563,49 +604,49 @@ template <typename T, int N> constexpr int variant_P(int v) {
 
 template <typename T, int N, int P>
 static void run_fast(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
-                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done, const int *slice_map) {
   const unsigned nb = (unsigned)((ax.npencil + P - 1) / P);
   hipLaunchKernelGGL((k_fft_fast<T, N, P>), dim3(nb, nfield, nslice), dim3(kFftThreads), 0, s, (vec2<T> *)F,
-                     fstride, ax.S, ax.npencil, ax.tw, mode, ps, ctrl, slice_len, slice_done);
+                     fstride, ax.S, ax.npencil, ax.tw, mode, ps, ctrl, slice_len, slice_done, slice_map);
 }
 
 template <typename T, int N>
 static bool try_fast(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
-                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done, const int *slice_map) {
   if (ax.N != N) return false;
   // the fused p^2 pass runs on a strided axis (it is always the last one)
   if (mode == 2 && ax.S == 1) return false;
   if (ax.pv == 1)
-    run_fast<T, N, variant_P<T, N>(1)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
+    run_fast<T, N, variant_P<T, N>(1)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map);
   else if (ax.pv == 2)
-    run_fast<T, N, variant_P<T, N>(2)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
+    run_fast<T, N, variant_P<T, N>(2)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map);
   else
-    run_fast<T, N, variant_P<T, N>(0)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done);
+    run_fast<T, N, variant_P<T, N>(0)>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map);
   return true;
 }
 
 template <typename T>
 void launch_fft_axis(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
-                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done) {
-  if (try_fast<T, 4>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 6>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 8>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 12>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 16>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 24>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 32>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 48>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 64>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done) ||
-      try_fast<T, 96>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done))
+                     const Ctrl *ctrl, hipStream_t s, int nslice, long long slice_len, const int *slice_done, const int *slice_map) {
+  if (try_fast<T, 4>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 6>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 8>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 12>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 16>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 24>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 32>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 48>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 64>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map) ||
+      try_fast<T, 96>(F, fstride, nfield, ax, mode, ps, ctrl, s, nslice, slice_len, slice_done, slice_map))
     return;
   const unsigned nb = (unsigned)((ax.npencil + ax.P - 1) / ax.P);
   const size_t sm = (size_t)ax.N * (ax.P + 1) * sizeof(vec2<T>);
   hipLaunchKernelGGL(k_fft_axis<T>, dim3(nb, nfield, nslice), dim3(kFftThreads), sm, s, (vec2<T> *)F, fstride, ax,
-                     mode, ps, ctrl, slice_len, slice_done);
+                     mode, ps, ctrl, slice_len, slice_done, slice_map);
 }
 template void launch_fft_axis<double>(void *, long long, int, const FftAxis &, int, const PsqInfo &, const Ctrl *,
-                                      hipStream_t, int, long long, const int *);
+                                      hipStream_t, int, long long, const int *, const int *);
 template void launch_fft_axis<float>(void *, long long, int, const FftAxis &, int, const PsqInfo &, const Ctrl *,
-                                     hipStream_t, int, long long, const int *);
+                                     hipStream_t, int, long long, const int *, const int *);
 
 }  // namespace gfix

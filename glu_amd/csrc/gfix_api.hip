@@ -191,6 +191,7 @@ extern "C" int glu_gfix_create(glu_gfix_ctx **out, const size_t dims[4], int gft
     if (const char *e = getenv("GLU_GFIX_GRAPH")) c->use_graph = atoi(e);
     if (const char *e = getenv("GLU_GFIX_BATCH")) c->batch = std::max(1, std::min(64, atoi(e)));
     if (const char *e = getenv("GLU_GFIX_ZT")) c->use_zt = atoi(e);
+    if (const char *e = getenv("GLU_GFIX_ZT_PV")) c->zt_pv = std::min(std::max(atoi(e), 0), 2);
     CK(hipMalloc(&c->U64, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->U64b, (size_t)V * 24 * sizeof(double2)));
     CK(hipMalloc(&c->F, (size_t)V * 4 * sizeof(double2)));
@@ -363,10 +364,11 @@ Geo geo_order(const glu_gfix_ctx *c, int order) {
 
 // forward transform, p^2 preconditioner, backward transform of the 4 packed
 // fields: over the whole lattice (Landau) or per time slice (Coulomb)
-template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, const int *slice_done) {
+template <typename T>
+void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, const int *slice_done, const int *slice_map, int nmap) {
   const int D = c->ps.D;
   const long long V = c->g.V;
-  const int ns = D == 4 ? 1 : c->g.L[3];
+  const int ns = D == 4 ? 1 : (slice_map ? nmap : c->g.L[3]);
   const long long sl = D == 4 ? 0 : c->g.Lcu;
   // number of x-y planes per transform
   const long long nplane = (D == 4 ? V : c->g.Lcu) / c->g.S[2];
@@ -374,23 +376,26 @@ template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const Ctrl *ctrl, co
     bool ok = false;
     GFIX_PROF(c, pk<T>(PK_FFT),
               ok = c->use_xy && launch_fft_xy<T>(F, V, 4, c->ax[0], c->ax[1], mode, ctrl, c->stream, nplane, ns, sl,
-                                                  slice_done));
+                                                  slice_done, slice_map));
     return ok;
   };
   const bool fused = xy(0);
   if (fused && D == 4 && c->use_zt &&
-      launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream, false)) {
+      launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream, false, c->zt_pv)) {
     GFIX_PROF(c, pk<T>(PK_FFT),
-              launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream));
+              launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, ctrl, c->stream, true,
+                               c->zt_pv));
     xy(1);
     return;
   }
   for (int a = fused ? 2 : 0; a < D - 1; a++)
-    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 0, c->ps, ctrl, c->stream, ns, sl, slice_done));
+    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 0, c->ps, ctrl, c->stream, ns, sl, slice_done,
+                                                             slice_map));
   GFIX_PROF(c, pk<T>(PK_FFT),
-            launch_fft_axis<T>(F, V, 4, c->ax[D - 1], 2, c->ps, ctrl, c->stream, ns, sl, slice_done));
+            launch_fft_axis<T>(F, V, 4, c->ax[D - 1], 2, c->ps, ctrl, c->stream, ns, sl, slice_done, slice_map));
   for (int a = D - 2; a >= (fused ? 2 : 0); a--)
-    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 1, c->ps, ctrl, c->stream, ns, sl, slice_done));
+    GFIX_PROF(c, pk<T>(PK_FFT), launch_fft_axis<T>(F, V, 4, c->ax[a], 1, c->ps, ctrl, c->stream, ns, sl, slice_done,
+                                                             slice_map));
   if (fused) xy(1);
 }
 hipEvent_t prof_event(glu_gfix_ctx *c) {
@@ -413,8 +418,8 @@ void prof_harvest(glu_gfix_ctx *c) {
   }
   c->prof_pending.clear();
 }
-template void enq_fa<double>(glu_gfix_ctx *, void *, const Ctrl *, const int *);
-template void enq_fa<float>(glu_gfix_ctx *, void *, const Ctrl *, const int *);
+template void enq_fa<double>(glu_gfix_ctx *, void *, const Ctrl *, const int *, const int *, int);
+template void enq_fa<float>(glu_gfix_ctx *, void *, const Ctrl *, const int *, const int *, int);
 
 static double landau_norm(const glu_gfix_ctx *c) { return 1.0 / (3.0 * (double)c->g.V); }
 

@@ -77,6 +77,7 @@ struct glu_gfix_ctx {
   double stall_theta = 1e-6, stall_factor = 0.0;
   int use_xy = 1;     // x-y plane FFT passes where the plane fits a workgroup
   int use_zt = 1;     // Landau: z and t axes with p^2 in one pass after the x-y pass
+  int zt_pv = 0;      // x values per workgroup of that pass (variant, tuned)
   // workgroup order (Geo::xcd) of the derivative / fused update and of the
   // line search; set by the autotuner
   int ord_eg = 1, ord_ls = 1;
@@ -151,7 +152,9 @@ void prof_harvest(glu_gfix_ctx *c);
 
 int gfix_fail(int code, const char *fmt, ...);
 size_t gfix_footprint(const size_t dims[4], int gftype, int precision);
-template <typename T> void enq_fa(glu_gfix_ctx *c, void *F, const gfix::Ctrl *ctrl, const int *slice_done);
+template <typename T>
+void enq_fa(glu_gfix_ctx *c, void *F, const gfix::Ctrl *ctrl, const int *slice_done, const int *slice_map = nullptr,
+            int nmap = 0);
 
 int coulomb_alloc(glu_gfix_ctx *c);
 void coulomb_free(glu_gfix_ctx *c);

@@ -31,26 +31,29 @@ struct PsqInfo {
 };
 
 // nslice > 1: independent transforms of slice_len elements each (per field),
-// ax.npencil counting the pencils of one slice
+// ax.npencil counting the pencils of one slice; with slice_map, batch row z
+// transforms slice slice_map[z] (nslice rows); slices flagged in slice_done
+// are skipped
 template <typename T>
 void launch_fft_axis(void *F, long long fstride, int nfield, const FftAxis &ax, int mode, const PsqInfo &ps,
                      const Ctrl *ctrl, hipStream_t s, int nslice = 1, long long slice_len = 0,
-                     const int *slice_done = nullptr);
+                     const int *slice_done = nullptr, const int *slice_map = nullptr);
 
 // both in-plane axes (x, y) in one pass when the plane fits a workgroup;
 // false when this shape has no plane kernel (the caller runs axis passes)
 template <typename T>
 bool launch_fft_xy(void *F, long long fstride, int nfield, const FftAxis &ax, const FftAxis &ay, int mode,
                    const Ctrl *ctrl, hipStream_t s, long long nplane, int nslice, long long slice_len,
-                   const int *slice_done);
+                   const int *slice_done, const int *slice_map = nullptr);
 
 // the z and t axes of the 4D transform with p^2 in between (forward z, t,
 // x p^2, backward t, z) in one pass over the fields; false when this shape
 // has no z-t kernel (the caller runs axis passes); launch = false only
-// answers whether it has one
+// answers whether it has one; variant: x values per workgroup (tuned)
 template <typename T>
 bool launch_fft_zt(void *F, long long V, int nfield, const FftAxis &ax, const FftAxis &ay, const FftAxis &az,
-                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch = true);
+                   const FftAxis &at, const PsqInfo &ps, const Ctrl *ctrl, hipStream_t s, bool launch = true,
+                   int variant = 0);
 
 template <typename T>
 void launch_deriv(const void *U, void *F, double *part, const Geo &g, const Ctrl *ctrl, hipStream_t s);

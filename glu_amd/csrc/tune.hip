@@ -3,6 +3,7 @@
 // The tunable shapes are those that change speed and nothing else:
 //   - pencils per workgroup of each LDS FFT axis pass (three variants),
 //   - the fused x-y plane pass against two separate axis passes,
+//   - the x values per workgroup of the z-t column pass,
 //   - the workgroup order of the fused update and of the line search
 //     (plain, XCD-contiguous, XCD time slabs).
 // glu_gfix_autotune times every candidate with HIP events on the context's
@@ -26,6 +27,7 @@ namespace {
 struct Tuning {
   int pv[4];
   int use_xy;
+  int zt_pv;
   int ord_eg, ord_ls;
   int ls_tile;
   std::string report;
@@ -34,11 +36,18 @@ struct Tuning {
 std::mutex g_tune_mu;
 std::map<std::tuple<size_t, size_t, size_t, size_t, int, int>, Tuning> g_tune_cache;
 
-// mean duration of `reps` launches after one untimed launch
+// mean duration of `reps` launches after one untimed launch; a candidate
+// whose launch fails (a shape the device refuses) times as 1e30
 template <class Fn> float time_ms(glu_gfix_ctx *c, Fn &&launch, int reps) {
   hipEvent_t a = nullptr, b = nullptr;
   if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return 1e30f;
+  (void)hipGetLastError();  // clear an earlier, already reported error
   launch();
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    return 1e30f;
+  }
   (void)hipEventRecord(a, c->stream);
   for (int r = 0; r < reps; r++) launch();
   (void)hipEventRecord(b, c->stream);
@@ -93,6 +102,26 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
     snprintf(line, sizeof(line), "fft x-y plane pass: %.4f ms against %.4f ms for two passes -> %s\n", xy,
              best_axis[0] + best_axis[1], t.use_xy ? "plane" : "axes");
     t.report += line;
+  }
+  // the z-t column pass (Landau, after a plane pass): x values per workgroup
+  t.zt_pv = 0;
+  if (t.use_xy && D == 4 &&
+      launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, nullptr, c->stream, false, 0)) {
+    float best = 1e30f;
+    for (int v = 0; v < 3; v++) {
+      const float ms = time_ms(
+          c,
+          [&] {
+            launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, nullptr, c->stream, true, v);
+          },
+          kReps);
+      snprintf(line, sizeof(line), "fft z-t column pass variant %d: %.4f ms\n", v, ms);
+      t.report += line;
+      if (ms < best) {
+        best = ms;
+        t.zt_pv = v;
+      }
+    }
   }
   // workgroup order of the two stencil kernels that dominate an iteration,
   // each timed on its own with a private control block (not converged,
@@ -158,6 +187,7 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
 void apply(glu_gfix_ctx *c, const Tuning &t) {
   for (int a = 0; a < 4; a++) c->ax[a].pv = t.pv[a];
   c->use_xy = t.use_xy;
+  c->zt_pv = t.zt_pv;
   c->ord_eg = t.ord_eg;
   c->ord_ls = t.ord_ls;
   c->ls_tile = t.ls_tile;

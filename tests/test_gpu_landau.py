@@ -92,6 +92,23 @@ def test_stage_fourier_accelerate(dims):
     assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("dims", [(16, 16, 16, 32), (32, 32, 32, 64)])
+def test_stage_fourier_accelerate_zt_variants(dims, variant, monkeypatch):
+    """The z-t column pass with the other x-values-per-workgroup variants
+    the autotuner may pick."""
+    monkeypatch.setenv("GLU_GFIX_ZT_PV", str(variant))
+    rng = np.random.default_rng(6)
+    V = int(np.prod(dims))
+    F = rng.standard_normal((4, V)) + 1j * rng.standard_normal((4, V))
+    ref = O.Lattice(dims, nth=2).fourier_accelerate(F)
+    U = np.zeros((V, 4, 18))
+    U[..., [0, 8, 16]] = 1.0
+    with fresh(dims, U) as g:
+        out = g.stage_fa(F)
+    assert np.abs(out - ref).max() < 1e-13 * np.abs(ref).max()
+
+
 def test_stage_line_search_and_egauge(cfg):
     U, _ = cfg
     L = O.Lattice(DIMS, nth=4)
