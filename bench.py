@@ -81,6 +81,25 @@ def parse():
     return ap.parse_args()
 
 
+_JSON_OUT = None
+
+
+def route_stdout():
+    """Keep the process's stdout for the one JSON line and send everything
+    else written to file descriptor 1 -- the progress lines GLU's entry
+    points and the oracle print from C included -- to stderr."""
+    global _JSON_OUT
+    sys.stdout.flush()
+    _JSON_OUT = os.fdopen(os.dup(1), "w")
+    os.dup2(2, 1)
+
+
+def emit(line):
+    out = _JSON_OUT or sys.stdout
+    out.write(json.dumps(line) + "\n")
+    out.flush()
+
+
 def pct(v, q):
     v = sorted(v)
     k = (len(v) - 1) * q
@@ -147,6 +166,7 @@ def main():
               "configuration fits one MI355X and the multi-GPU design (one config "
               "per GPU, no communication) has no figure to measure here.", file=sys.stderr)
         sys.exit(2)
+    route_stdout()
     import glu_amd as gl
 
     if a.workload == "coulomb":
@@ -258,7 +278,7 @@ def main():
         "mixed_speedup_vs_fp64": round(mixed_value / fp64["configs_per_s"], 3) if fp64 else None,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def roofline(kname, prec, V, mean_ms, dims):
@@ -406,7 +426,7 @@ def coulomb_main(a):
         "max_slice_theta": float(max(sth)), "plaquette_drift": abs(m["plaquette"] - plaq),
         "stream_GBps": round(gbs, 1), "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 def coulomb_cpu_baseline(g, dims):
@@ -515,7 +535,7 @@ def glueprop_main(a):
                      "unit": "GB/s", "frac": round(ach / PEAK_HBM_GBS, 4), "traffic": traffic},
         "kernels": kern, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 SERVE_SIZES = ((16, 16, 16, 32), (24, 24, 24, 48), (32, 32, 32, 64))
@@ -632,7 +652,7 @@ def serve_main(a):
                    "switch_theta": a.switch_theta, "slots": a.slots},
         "p50_ms": lat["p50_ms"], "p99_ms": lat["p99_ms"], "runs": runs,
     }
-    print(json.dumps(line), flush=True)
+    emit(line)
 
 
 if __name__ == "__main__":

@@ -39,3 +39,23 @@ def test_iteration_bytes_are_the_sum_of_the_steps():
     # FP32 streaming moves half the bytes of FP64
     for k, (b64, b32) in b.ALG_BYTES.items():
         assert b64 == 2 * b32, k
+
+
+def test_stdout_carries_only_the_json_line():
+    """Progress lines written to file descriptor 1 from C (GLU's [GF]/[CG]
+    lines of the library and the oracle) and from Python go to stderr once
+    the bench has routed its output; stdout keeps the one JSON line."""
+    code = (
+        "import importlib.util, os, sys, ctypes\n"
+        f"spec = importlib.util.spec_from_file_location('bench', {os.path.join(ROOT, 'bench.py')!r})\n"
+        "b = importlib.util.module_from_spec(spec); spec.loader.exec_module(b)\n"
+        "b.route_stdout()\n"
+        "libc = ctypes.CDLL(None)\n"
+        "libc.printf(b'[CG] random gauge transform 0 :: 1e-09 (650)\\n'); libc.fflush(None)\n"
+        "print('[bench] progress', flush=True)\n"
+        "b.emit({'metric': 'm', 'value': 1.0})\n"
+    )
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout == '{"metric": "m", "value": 1.0}\n'
+    assert "[CG] random gauge transform" in r.stderr and "[bench] progress" in r.stderr
