@@ -9,12 +9,13 @@ a device heatbath config (beta = 6.0, cold start, 4 overrelaxation passes
 per sweep) with a seeded random gauge transform, generated once before the
 timed region.
 
-Prints one JSON line (see the keys below).  Three further workloads print a
+Prints one JSON line (see the keys below).  Four further workloads print a
 line of their own: --workload coulomb (48^3x96 Coulomb FACG, the 96 time
 slices batched), --workload serve (a Poisson stream of 64 Landau
-requests of mixed sizes through the serving loop) and --workload glueprop
+requests of mixed sizes through the serving loop), --workload glueprop
 (gluon propagators of a Landau-fixed 32^3x64 config, GLU's GLUON_PROPS
-cuts).
+cuts) and --workload batch (512 gauge copies of an 8^4 config fixed in one
+launch, one workgroup per lattice).
 """
 from __future__ import annotations
 
@@ -33,6 +34,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("Landau configs/s to theta<1e-14 at 32^3x64 (mixed vs FP64); "
           "p50/p99 latency; GB/s")
 PEAK_HBM_GBS = 8000.0
+SENTINEL = 123456789  # FACG()'s failure return
 # algorithmic bytes per site per launch (FP64 / FP32 streaming) of each
 # kernel: each logical step reads its inputs once and writes its outputs
 # once; an FFT axis pass reads and writes the 4 packed fields
@@ -68,7 +70,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="CPU baseline sample: k and 2k oracle iterations")
-    ap.add_argument("--workload", choices=("landau", "coulomb", "serve", "glueprop"), default="landau")
+    ap.add_argument("--workload", choices=("landau", "coulomb", "serve", "glueprop", "batch"), default="landau")
+    ap.add_argument("--batch", type=int, default=512, help="batch: lattices per launch")
     ap.add_argument("--max-mom", type=int, default=64, help="glueprop: PSQ cut p^2 <= max_mom")
     ap.add_argument("--graph", action="store_true", help="replay iteration batches from HIP graphs")
     ap.add_argument("--requests", type=int, default=64)
@@ -175,6 +178,8 @@ def main():
         return serve_main(a)
     if a.workload == "glueprop":
         return glueprop_main(a)
+    if a.workload == "batch":
+        return batch_main(a)
     dims = tuple(int(x) for x in a.dims.split(","))
     V = math.prod(dims)
     print(f"[bench] lattice {dims} V={V}: generating input "
@@ -345,6 +350,144 @@ COULOMB_KERNEL_BYTES = {
     "c_line_search": (576 + 144 + 64, 288 + 96 + 64), "c_slice_update": (144 + 64 + 144, 96 + 64 + 96),
     "fft_axis": (128, 128),
 }
+
+
+def batch_main(a):
+    """Landau FACG of many small lattices of one shape, one workgroup per
+    lattice, all in one launch (glu_gfix_facg_batch), FP64, against the
+    one-lattice context path on the same gauge copies.  One step = restore
+    the B inputs from their device copy + one launch over all of them, with
+    Landau()'s random restarts (on the context path) for any that fail."""
+    import torch
+
+    import glu_amd as gl
+
+    dims = (8, 8, 8, 8) if a.dims == "32,32,32,64" else tuple(int(x) for x in a.dims.split(","))
+    V, B = math.prod(dims), a.batch
+    g = gl.Gfix(dims, gl.LANDAU, gl.FP64)
+    t = time.perf_counter()
+    g.heatbath(6.0, a.sweeps, 4, seed=a.seed, cold_start=True)
+    plaq = g.measure()["plaquette"]
+    g.snapshot()
+    src = torch.empty((B, V * 72), dtype=torch.float64, device="cuda")
+    for k in range(B):  # gauge copy k: the input under random transform seed + 1 + k
+        g.restore()
+        g.random_transform(seed=a.seed + 1 + k)
+        g.get_links_device(src[k].data_ptr())
+    torch.cuda.synchronize()
+    dev = torch.empty_like(src)
+    print(f"[bench] batch {dims} x {B}: inputs ready in {time.perf_counter() - t:.1f}s, plaquette {plaq:.12f}",
+          file=sys.stderr, flush=True)
+
+    def step():
+        dev.copy_(src)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        its, th = gl.facg_batch(dims, dev.data_ptr(), B, 0.08, 1e-14, 1000)
+        t_launch = time.perf_counter() - t1
+        failed = [k for k, x in enumerate(its) if x == SENTINEL or not th[k] < 1e-14]
+        for k in failed:  # Landau()'s restarts, from the lattice's input
+            g.set_links_device(src[k].data_ptr())
+            it, thk, _ = g.landau(gl.CG, 0.08, 1e-14, 1000, restart_seed=a.seed + 1 + k)
+            if not thk < 1e-14:
+                raise RuntimeError(f"lattice {k} did not converge: theta {thk}")
+            g.get_links_device(dev[k].data_ptr())
+            its[k] = it
+        torch.cuda.synchronize()
+        return its, t_launch, len(failed)
+
+    for _ in range(max(a.warmup, 1)):
+        step()
+    lat, launch, its_all, restarts = [], [], [], 0
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        its, tl, nf = step()
+        lat.append(time.perf_counter() - t)
+        launch.append(tl)
+        its_all += its
+        restarts += nf
+    total = time.perf_counter() - t0
+    g.set_links_device(dev[0].data_ptr())  # lattice 0 as fixed by the last step
+    g.reunit()
+    drift = abs(g.measure()["plaquette"] - plaq)
+    # the context path on the first copies, one after another
+    nctx = min(B, 16)
+    ctx_its = []
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for k in range(nctx):
+        g.set_links_device(src[k].data_ptr())
+        it, thk, _ = g.landau(gl.CG, 0.08, 1e-14, 1000, restart_seed=a.seed + 1 + k)
+        ctx_its.append(it)
+    g.measure()
+    t_ctx = time.perf_counter() - t
+    # the serving loop on the same copies, every request submitted at once:
+    # with shared launches, and one context per request (8 in flight)
+    served = {}
+    for batching in (True, False):
+        n = B if batching else min(B, 128)
+        bufs = src[:n].clone()
+        torch.cuda.synchronize()
+        with gl.Server(max_active=8, batching=batching) as srv:
+            t = time.perf_counter()
+            tickets = [srv.submit(dims, bufs[k].data_ptr(), gl.LANDAU, gl.FP64) for k in range(n)]
+            for tk in tickets:
+                srv.wait(tk)
+            dt = time.perf_counter() - t
+        served["shared_launches" if batching else "contexts"] = {"requests": n, "configs_per_s": round(n / dt, 1)}
+        print(f"[bench] server, {'shared launches' if batching else 'contexts'}: {n} requests in {dt:.2f}s",
+              file=sys.stderr, flush=True)
+    n_ok = [x for x in its_all if x < SENTINEL]
+    mean_it = sum(n_ok) / max(len(n_ok), 1)
+    # each lattice iteration moves the logical bytes of one FACG iteration
+    # of its V sites (links as GLU stores them, 9 complex numbers)
+    launch_s = sum(launch) / len(launch)
+    gbs = sum(its_all) / a.steps * V * ALG_ITER[0] / launch_s / 1e9 if not restarts else None
+    cpu = None if a.no_cpu else batch_cpu_baseline(src, dims, B)
+    line = {
+        "metric": f"Landau configs/s to theta<1e-14 at {'x'.join(map(str, dims))}, {B} lattices per launch",
+        "value": round(B * a.steps / total, 2), "unit": "configs/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+        "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, 4 overrelaxation "
+                 f"each, seed {a.seed}); lattice k of a batch is its gauge copy under random transform "
+                 f"seed {a.seed + 1}+k"),
+        "config": {"workload": "landau_facg_batch_fp64_" + "x".join(map(str, dims)), "lattice": list(dims),
+                   "lattices_per_launch": B, "gf_tune": 0.08, "accuracy": 1e-14, "max_iters": 1000},
+        "p50_ms": round(1e3 * pct(lat, 0.5), 2), "launch_ms": round(1e3 * launch_s, 2),
+        "iters_mean": round(mean_it, 1), "iters_max": max(n_ok) if n_ok else None, "restarts": restarts,
+        "plaquette_drift_lattice0": drift,
+        "context_path": {"configs_per_s": round(nctx / t_ctx, 2), "iters_mean": round(sum(ctx_its) / nctx, 1),
+                         "copies": nctx},
+        "speedup_vs_context_path": round((B * a.steps / total) / (nctx / t_ctx), 2),
+        "server": served,
+        "stream_GBps": round(gbs, 1) if gbs else None,
+        "roofline": {"kernel": "k_facg_batch (whole FACG per workgroup)", "bound": "hbm",
+                     "achieved": round(gbs, 1) if gbs else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(gbs / PEAK_HBM_GBS, 4) if gbs else None, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    emit(line)
+
+
+def batch_cpu_baseline(src, dims, B):
+    """The CPU oracle (C, OpenMP) solving gauge copies of the batch one
+    after another, until about 10 s have passed."""
+    from oracle import oracle as O
+
+    nth = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    L = O.Lattice(dims, nth=nth)
+    t = time.perf_counter()
+    n, its = 0, []
+    while n < B and (time.perf_counter() - t < 10.0 or n == 0):
+        W = src[n].cpu().numpy().reshape(-1, 4, 18).copy()
+        its.append(L.facg(W, 0.08, 1e-14, 1000)["iters"])
+        n += 1
+    dt = time.perf_counter() - t
+    return {"value": round(n / dt, 3), "unit": "configs/s", "cores": nth, "kind": "port",
+            "sample": (f"oracle FP64 FACG on the first {n} gauge copies of the batch, one after another "
+                       f"({dt:.1f}s, iterations {min(its)}..{max(its)})")}
 
 
 def coulomb_main(a):

@@ -33,11 +33,11 @@ EXPORTS = [
     "glu_gfix_last_error", "glu_gfix_heatbath", "glu_gfix_random_transform",
     "glu_gfix_profile", "glu_gfix_profile_read", "glu_gfix_set_switch_theta",
     "glu_gfix_get_links_device", "glu_gfix_set_graph", "glu_gfix_landau", "glu_gfix_autotune", "glu_gfix_luxury",
-    "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_submit",
+    "glu_gfix_server_create", "glu_gfix_server_configure", "glu_gfix_server_set_batching", "glu_gfix_server_submit",
     "glu_gfix_server_wait", "glu_gfix_server_stats", "glu_gfix_server_destroy", "glu_gfix_write_nersc",
     "glu_gfix_gluon_props", "glu_gfix_gluon_exceptional", "glu_gfix_rng_seed", "glu_gfix_glu_random_transform",
     "glu_gfix_rng_draws", "glu_gfix_phase_ms", "glu_gfix_server_submit_glu", "glu_gfix_measure_noalloc",
-    "glu_gfix_slice_trace", "glu_gfix_glu_heatbath",
+    "glu_gfix_slice_trace", "glu_gfix_glu_heatbath", "glu_gfix_facg_batch",
 ]
 
 KERNELS = ["deriv", "fft_axis", "pr_sums", "cg_update", "line_search", "egauge", "egauge_deriv", "lie_pack",
@@ -96,6 +96,7 @@ def lib():
                                           C.POINTER(SZ), C.POINTER(C.c_int)]),
             "glu_gfix_server_create": (C.c_int, [C.POINTER(P), SZ, C.c_int]),
             "glu_gfix_server_configure": (C.c_int, [P, C.c_double, C.c_int, C.c_uint64]),
+            "glu_gfix_server_set_batching": (C.c_int, [P, C.c_int]),
             "glu_gfix_landau": (C.c_int, [P, C.c_int, C.c_double, C.c_double, SZ, C.c_uint64, DP,
                                           C.POINTER(SZ), C.POINTER(C.c_int)]),
             "glu_gfix_server_submit": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_int, C.c_int, C.c_double,
@@ -111,6 +112,8 @@ def lib():
             "glu_gfix_server_submit_glu": (C.c_int, [P, C.POINTER(SZ), C.c_int, C.c_char_p, C.c_void_p,
                                                      C.POINTER(C.c_int)]),
             "glu_gfix_measure_noalloc": (C.c_int, [C.c_void_p, C.POINTER(SZ), C.c_void_p, C.c_void_p]),
+            "glu_gfix_facg_batch": (C.c_int, [C.POINTER(SZ), SZ, C.c_void_p, C.c_double, C.c_double, SZ,
+                                              C.POINTER(SZ), DP, DP, SZ]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -357,6 +360,20 @@ def rng_draws(seed: int, nthreads: int, thread: int, n: int) -> np.ndarray:
     return out
 
 
+def facg_batch(dims, ptr: int, nlat: int, gf_tune=0.08, accuracy=1e-14, max_iters=1000, trace_cap=0):
+    """FACG() of nlat lattices of one small shape in one launch (device
+    buffer `ptr`: nlat x V x 72 doubles in GLU's layout, fixed in place).
+    Returns (iters, theta[, trace rows per lattice])."""
+    d = (SZ * 4)(*dims)
+    it = (SZ * nlat)()
+    th = np.zeros(nlat)
+    tr = np.zeros((nlat, max(trace_cap, 1), 3))
+    _check(lib().glu_gfix_facg_batch(d, nlat, C.c_void_p(ptr), gf_tune, accuracy, max_iters, it, _dp(th),
+                                     _dp(tr) if trace_cap else None, trace_cap), "facg_batch")
+    its = [int(x) for x in it]
+    return (its, th, tr) if trace_cap else (its, th)
+
+
 def footprint(dims, gftype=LANDAU, precision=FP64) -> int:
     return int(lib().glu_gfix_footprint((SZ * 4)(*dims), gftype, precision))
 
@@ -366,12 +383,14 @@ class Server:
     pointers to GLU-layout links, fixed in place."""
 
     def __init__(self, hbm_budget: int = 0, max_active: int = 8, switch_theta: float = 0.0,
-                 use_graph: bool = True, restart_seed: int = 1):
+                 use_graph: bool = True, restart_seed: int = 1, batching: bool = False):
         h = C.c_void_p()
         _check(lib().glu_gfix_server_create(C.byref(h), hbm_budget, max_active), "server_create")
         self._h = h
         _check(lib().glu_gfix_server_configure(h, switch_theta, int(bool(use_graph)), restart_seed),
                "server_configure")
+        if batching:  # small FP64 Landau lattices in shared launches
+            _check(lib().glu_gfix_server_set_batching(h, 1), "server_set_batching")
 
     def submit(self, dims, ptr: int, gftype=LANDAU, precision=MIXED, method=CG, gf_tune=0.08,
                accuracy=1e-14, max_iters=1000) -> int:

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libglu_gfix.so")
 SOURCES = ["landau_kernels.hip", "fft_kernels.hip", "gfix_api.hip", "coulomb.hip", "glu_compat.hip", "inputs.hip",
-           "serve.hip", "tune.hip", "nersc_io.hip", "glueprop.hip", "mwc.hip", "noalloc.hip", "glu_hb.hip"]
+           "serve.hip", "tune.hip", "nersc_io.hip", "glueprop.hip", "mwc.hip", "noalloc.hip", "glu_hb.hip", "facg_batch.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GLU_GFIX_ARCH", "gfx950")
 

@@ -163,6 +163,12 @@ int landau_fix(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_it
 int landau_begin(glu_gfix_ctx *c, int method, double gf, double acc, size_t max_iters);
 int landau_advance(glu_gfix_ctx *c, int block);
 int landau_restart_policy(size_t first, size_t loc, size_t max_iters, size_t *iters);
+// FACG of nlat small lattices of one shape in one launch (links at the
+// device pointers lat[0..nlat)), as glu_gfix_facg_batch; with keep_failed
+// a lattice that fails keeps its input links
+int facg_batch_run(const size_t dims[4], size_t nlat, double *const *lat, double gf_tune, double accuracy,
+                   size_t max_iters, size_t *iters, double *theta, double *trace, size_t trace_cap,
+                   int keep_failed);
 gfix::Geo geo_order(const glu_gfix_ctx *c, int order);
 extern "C" int glu_gfix_random_transform(glu_gfix_ctx *c, uint64_t seed);
 extern "C" int glu_gfix_measure(glu_gfix_ctx *c, double *plaq, double *tr, double *mx, double *func, double *tlink);

@@ -131,6 +131,17 @@ template <typename T> struct DerivAcc {
   }
 };
 
+// parabola vertex through the three probes, clamped: Gfix/line_search.c:38-57
+__device__ __forceinline__ double approx_minimum(const double al[3], const double f[3]) {
+  const double A = f[0] / ((al[0] - al[1]) * (al[0] - al[2]));
+  const double B = f[1] / ((al[1] - al[0]) * (al[1] - al[2]));
+  const double C = f[2] / ((al[2] - al[0]) * (al[2] - al[1]));
+  const double sol = (al[0] * (B + C) + al[1] * (A + C) + al[2] * (A + B)) / (2 * (A + B + C));
+  if (sol > al[2]) return al[2];
+  if (isnan(sol) || isinf(sol) || sol < al[0]) return 0.08;
+  return sol;
+}
+
 // Device-resident scalars of one FACG/FASD run.  Every per-iteration kernel
 // reads `done` first and returns when it is set, so a captured batch of
 // iterations becomes a no-op past convergence.

@@ -481,17 +481,6 @@ __global__ void k_fin_sd(const double *__restrict__ part, long long nb, Ctrl *ct
   }
 }
 
-// parabola vertex through the three probes, clamped: Gfix/line_search.c:38-57
-__device__ __forceinline__ double approx_minimum(const double al[3], const double f[3]) {
-  const double A = f[0] / ((al[0] - al[1]) * (al[0] - al[2]));
-  const double B = f[1] / ((al[1] - al[0]) * (al[1] - al[2]));
-  const double C = f[2] / ((al[2] - al[0]) * (al[2] - al[1]));
-  const double sol = (al[0] * (B + C) + al[1] * (A + C) + al[2] * (A + B)) / (2 * (A + B + C));
-  if (sol > al[2]) return al[2];
-  if (isnan(sol) || isinf(sol) || sol < al[0]) return 0.08;
-  return sol;
-}
-
 __global__ void k_fin_ls(const double *__restrict__ part, long long nb, Ctrl *ctrl, double *trace, int force,
                          double pl1, double pl2) {
   if (ctrl->done) return;

@@ -14,6 +14,13 @@
 // with: every context owns its buffers and every reduction is
 // deterministic.
 //
+// With glu_gfix_server_set_batching, FP64 Landau CG requests of small
+// lattices (every extent 2..16) skip the contexts: every queued request of
+// one shape and settings is fixed in one shared launch, one workgroup per
+// lattice (facg_batch.hip), on a host thread of its own; a lattice whose
+// FACG fails goes back to the queue head and takes Landau()'s restarts on
+// a context.
+//
 // Landau requests follow Landau()'s restart policy (glu_gfix_landau).
 // Coulomb requests (the batched-slice driver is synchronous) run on a host
 // thread of their own, so the worker keeps advancing the Landau runs in
@@ -67,6 +74,10 @@ struct Request {
   uint32_t seed = 0;
   // a Coulomb run on its own thread: 1 while it runs (worker-visible)
   std::shared_ptr<std::atomic<int>> async;
+  int ticket = 0;
+  // FACG failed in a shared small-lattice launch: the restarts of Landau()
+  // follow on a context, from the input the launch left untouched
+  int from_batch = 0;
   int attempt = 0;   // Landau() restarts taken so far
   size_t total = 0;  // iterations counted across restarts
   int rc = 0;
@@ -90,6 +101,20 @@ bool same_shape(const size_t a[4], const size_t b[4]) {
   return a[0] == b[0] && a[1] == b[1] && a[2] == b[2] && a[3] == b[3];
 }
 
+// extents the shared small-lattice launch takes (facg_batch.hip)
+constexpr size_t kBatchMaxExtent = 16;
+constexpr size_t kBatchMaxLattices = 65535;
+// its scratch per site of each lattice: 36 complex numbers
+constexpr size_t kBatchScratchPerSite = 36 * 16;
+
+// a shared launch in flight: its requests and the flag its host thread
+// clears when the results are in
+struct Batch {
+  std::vector<Request *> reqs;
+  std::shared_ptr<std::atomic<int>> running;
+  size_t bytes = 0;
+};
+
 }  // namespace
 
 struct glu_gfix_server {
@@ -99,6 +124,7 @@ struct glu_gfix_server {
   double switch_theta = 0.0;  // 0: the context default
   int use_graph = 1;
   uint64_t restart_seed = 1;
+  int batch_small = 0;  // small lattices in shared launches
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::map<int, Request> reqs;
@@ -113,6 +139,15 @@ struct glu_gfix_server {
 };
 
 namespace {
+
+bool batchable(const glu_gfix_server *s, const Request &r) {
+  if (!s->batch_small || r.gftype != GLU_GFIX_LANDAU || r.method != GLU_GFIX_CG || r.precision != GLU_GFIX_FP64 ||
+      r.random_transform || r.from_batch)
+    return false;
+  for (int mu = 0; mu < 4; mu++)
+    if (r.dims[mu] < 2 || r.dims[mu] > kBatchMaxExtent) return false;
+  return true;
+}
 
 // context for a request: an idle pooled one of the same shape, else a new
 // one once idle contexts of other shapes have made room.  Called with the
@@ -183,6 +218,7 @@ void complete(glu_gfix_server *s, Request &r, int rc, const std::string &err) {
   s->cv_done.notify_all();
 }
 
+int after_attempt(Request &r, size_t loc, double theta);
 int after_run(Request &r);
 int copy_out(Request &r);
 
@@ -218,6 +254,12 @@ int start(glu_gfix_server *s, Request &r, bool *finished) {
     }).detach();
     return 0;
   }
+  if (r.from_batch) {  // attempt 0 failed in a shared launch: the restarts
+    const int more = after_attempt(r, 123456789, r.theta);
+    if (more < 0) return more;
+    *finished = more == 0;
+    return 0;
+  }
   if (int rc = landau_begin(c, r.method, r.gf, r.acc, r.max_iters)) return rc;
   if (c->run.finished) {
     const int more = after_run(r);
@@ -227,13 +269,12 @@ int start(glu_gfix_server *s, Request &r, bool *finished) {
   return 0;
 }
 
-// a Landau run ended: apply the restart policy of Landau()
-// (Gfix/Landau.c:301-345, as glu_gfix_landau).  0: the request is
-// finished, 1: a restarted run is in flight, < 0: error
-int after_run(Request &r) {
+// a Landau attempt ended with FACG()'s return loc: apply the restart
+// policy of Landau() (Gfix/Landau.c:301-345, as glu_gfix_landau).  0: the
+// request is finished, 1: a restarted run is in flight, < 0: error
+int after_attempt(Request &r, size_t loc, double theta) {
   glu_gfix_ctx *c = r.ctx;
-  const size_t loc = c->run.iters;
-  r.theta = c->run.theta;
+  r.theta = theta;
   if (r.attempt == 0) {
     r.total = loc;
     if (loc != 123456789) {
@@ -259,6 +300,8 @@ int after_run(Request &r) {
   return 1;
 }
 
+int after_run(Request &r) { return after_attempt(r, r.ctx->run.iters, r.ctx->run.theta); }
+
 // enqueue the copy of the fixed links into the request's buffer
 int copy_out(Request &r) {
   glu_gfix_ctx *c = r.ctx;
@@ -273,12 +316,56 @@ void worker_main(glu_gfix_server *s) {
   // requests this thread drives; std::map nodes stay put while other
   // threads insert, so the pointers stay valid until a request completes
   std::vector<Request *> active;
+  std::vector<Batch> batches;  // shared small-lattice launches in flight
   std::unique_lock<std::mutex> lk(s->mu);
+  // a request entering service takes the server's settings (lock held)
+  auto settle = [&](Request &r) {
+    r.state = RQ_RUNNING;
+    r.switch_theta = s->switch_theta;
+    r.use_graph = s->use_graph;
+    if (!r.seed) {
+      r.restart_seed = s->restart_seed;
+      r.seed = (uint32_t)s->restart_seed;
+    } else {
+      r.restart_seed = r.seed;
+    }
+  };
   for (;;) {
+    // small lattices: when the queue head can share a launch, every queued
+    // request of its shape and settings joins it, as many as the memory
+    // budget holds; a launch takes one slot
+    std::vector<Batch> launched;
+    while (!s->queue.empty() && (int)(active.size() + batches.size() + launched.size()) < s->max_active) {
+      const Request &h = s->reqs[s->queue.front()];
+      if (!batchable(s, h)) break;
+      const size_t hd[4] = {h.dims[0], h.dims[1], h.dims[2], h.dims[3]};
+      const double gf = h.gf, acc = h.acc;
+      const size_t mi = h.max_iters, per = hd[0] * hd[1] * hd[2] * hd[3] * kBatchScratchPerSite;
+      const size_t room = std::min(s->budget > s->used ? (s->budget - s->used) / per : 0, kBatchMaxLattices);
+      if (room == 0) break;  // waits for memory
+      Batch b;
+      for (auto it = s->queue.begin(); it != s->queue.end() && b.reqs.size() < room;) {
+        Request &r = s->reqs[*it];
+        if (batchable(s, r) && same_shape(r.dims, hd) && r.gf == gf && r.acc == acc && r.max_iters == mi) {
+          settle(r);
+          b.reqs.push_back(&r);
+          it = s->queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
+      b.bytes = b.reqs.size() * per;
+      s->used += b.bytes;
+      s->peak = std::max(s->peak, s->used);
+      b.running = std::make_shared<std::atomic<int>>(1);
+      launched.push_back(std::move(b));
+    }
     // admission: FIFO, as many as the budget and the slot count allow
     std::vector<Request *> admitted;
-    while (!s->queue.empty() && (int)(active.size() + admitted.size()) < s->max_active) {
+    while (!s->queue.empty() && (int)(active.size() + batches.size() + launched.size() + admitted.size()) <
+                                    s->max_active) {
       Request &r = s->reqs[s->queue.front()];
+      if (batchable(s, r)) break;  // waits for room for a shared launch
       int err = 0;
       glu_gfix_ctx *c = acquire(s, r, &err);
       if (err) {  // creation failed: fail the request, keep serving
@@ -292,19 +379,11 @@ void worker_main(glu_gfix_server *s) {
       }
       if (!c) break;  // waits for memory
       r.ctx = c;
-      r.state = RQ_RUNNING;
-      r.switch_theta = s->switch_theta;
-      r.use_graph = s->use_graph;
-      if (!r.seed) {
-        r.restart_seed = s->restart_seed;
-        r.seed = (uint32_t)s->restart_seed;
-      } else {
-        r.restart_seed = r.seed;
-      }
+      settle(r);
       admitted.push_back(&r);
       s->queue.pop_front();
     }
-    if (active.empty() && admitted.empty()) {
+    if (active.empty() && admitted.empty() && batches.empty() && launched.empty()) {
       if (s->stop && s->queue.empty()) break;
       s->cv_work.wait(lk);
       continue;
@@ -314,6 +393,55 @@ void worker_main(glu_gfix_server *s) {
     // running request
     std::vector<Ended> ended;
     auto fail = [&](Request *r, int rc) { ended.push_back({r, rc, glu_gfix_last_error()}); };
+    for (Batch &b : launched) {
+      // FACG of every lattice of the launch; a failed one keeps its input
+      // links for the restarts on a context
+      std::vector<double *> lat;
+      for (Request *r : b.reqs) lat.push_back(r->links);
+      const Request &h = *b.reqs[0];
+      std::thread([reqs = b.reqs, lat, dims = std::vector<size_t>(h.dims, h.dims + 4), gf = h.gf, acc = h.acc,
+                   mi = h.max_iters, flag = b.running, dev = s->device] {
+        (void)hipSetDevice(dev);
+        const size_t n = reqs.size();
+        std::vector<size_t> it(n);
+        std::vector<double> th(n);
+        const int rc = facg_batch_run(dims.data(), n, lat.data(), gf, acc, mi, it.data(), th.data(), nullptr, 0, 1);
+        const std::string err = rc ? glu_gfix_last_error() : "";
+        for (size_t k = 0; k < n; k++) {
+          Request *r = reqs[k];
+          r->rc = rc;
+          r->err = err;
+          if (!rc) {
+            r->theta = th[k];
+            r->iters = it[k];
+            r->from_batch = it[k] == 123456789;
+          }
+        }
+        flag->store(0, std::memory_order_release);
+      }).detach();
+      batches.push_back(std::move(b));
+    }
+    // finished shared launches: done, failed, or back to the queue head for
+    // Landau()'s restarts
+    std::vector<Request *> requeue;
+    size_t freed = 0;
+    for (auto it = batches.begin(); it != batches.end();) {
+      if (it->running->load(std::memory_order_acquire)) {
+        ++it;
+        continue;
+      }
+      for (Request *r : it->reqs) {
+        if (r->rc)
+          ended.push_back({r, r->rc, r->err});
+        else if (r->from_batch)
+          requeue.push_back(r);
+        else
+          ended.push_back({r, 0, std::string()});
+      }
+      freed += it->bytes;
+      it = batches.erase(it);
+    }
+    bool progressed = !admitted.empty() || !launched.empty() || freed;
     for (Request *r : admitted) {
       bool fin = false;
       int rc = start(s, *r, &fin);
@@ -323,7 +451,6 @@ void worker_main(glu_gfix_server *s) {
       else
         active.push_back(r);
     }
-    bool progressed = !admitted.empty();
     for (Request *r : active) {
       if (r->async) {  // a Coulomb run on its own thread
         if (r->async->load(std::memory_order_acquire)) continue;
@@ -367,9 +494,15 @@ void worker_main(glu_gfix_server *s) {
       }
     }
     lk.lock();
+    s->used -= freed;
+    for (auto r = requeue.rbegin(); r != requeue.rend(); ++r) {  // in their order, ahead of the rest
+      (*r)->state = RQ_QUEUED;
+      s->queue.push_front((*r)->ticket);
+    }
     for (Ended &e : ended) {
       complete(s, *e.r, e.rc, e.err);
-      active.erase(std::find(active.begin(), active.end(), e.r));
+      auto f = std::find(active.begin(), active.end(), e.r);
+      if (f != active.end()) active.erase(f);  // requests of a shared launch were never in it
     }
     if (ended.empty() && !progressed) {
       // every run in flight is busy on the device: poll again shortly
@@ -414,6 +547,14 @@ extern "C" int glu_gfix_server_configure(glu_gfix_server *s, double switch_theta
   return 0;
 }
 
+extern "C" int glu_gfix_server_set_batching(glu_gfix_server *s, int enable) {
+  if (!s) return gfix_fail(-1, "null server");
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->batch_small = enable ? 1 : 0;
+  s->cv_work.notify_one();
+  return 0;
+}
+
 static int submit(glu_gfix_server *s, const size_t dims[4], int gftype, int precision, int method, double gf_tune,
                   double accuracy, size_t max_iters, double *links_dev, int *ticket, uint32_t seed, int rtrans) {
   if (!s || !dims || !links_dev || !ticket) return gfix_fail(-1, "glu_gfix_server_submit: null argument");
@@ -440,6 +581,7 @@ static int submit(glu_gfix_server *s, const size_t dims[4], int gftype, int prec
   if (s->stop) return gfix_fail(-1, "server is shutting down");
   r.t_submit = Clock::now();
   const int t = s->next_ticket++;
+  r.ticket = t;
   s->reqs.emplace(t, r);
   s->queue.push_back(t);
   *ticket = t;

@@ -177,6 +177,28 @@ int glu_gfix_luxury(glu_gfix_ctx *ctx, int method, double gf_tune,
                     uint64_t seed, double *theta, size_t *iters,
                     int *best_copy);
 
+/* FACG() (Gfix/FACG.c:341-392, prototype Headers/FACG.h:37) for nlat small
+   lattices of one shape in one launch, one workgroup per lattice, FP64:
+   links_dev holds nlat configurations back to back in GLU's layout
+   (nlat x V x 72 doubles, device memory), each gauge fixed in place.  Per
+   lattice: iters[k] is FACG()'s return value (0 when theta_test_lin is
+   already below accuracy, the last segment's count after continuations
+   below 1e3 accuracy, 123456789 when it fails) and theta[k] the final
+   theta; with trace non-NULL, rows (theta_k, alpha_k, beta_k) as
+   glu_gfix_trace gives them, trace_cap rows per lattice (host memory).
+   Extents 2..16 (so at most 65536 sites), at most 65535 lattices; after
+   100 continuation segments a lattice returns 123456789 (GLU does not
+   bound them).  Landau()'s random restarts are left to the caller
+   (glu_gfix_landau on the failed ones).  Waits for the device's pending
+   work, runs on a stream of its own and returns when every lattice has
+   stopped; allocates 192 B per site per
+   lattice of device scratch for the call.  For lattices whose iterations
+   are bound by launch latency (8^4: one iteration of the context path
+   takes about 75 us). */
+int glu_gfix_facg_batch(const size_t dims[4], size_t nlat, double *links_dev, double gf_tune,
+                        double accuracy, size_t max_iters, size_t *iters, double *theta,
+                        double *trace, size_t trace_cap);
+
 /* Trajectory of the last run: rows (theta_k, alpha_k, beta_k); row 0 is the
    steepest-descent start.  Landau only.  Recording costs nothing extra. */
 int glu_gfix_trace(glu_gfix_ctx *ctx, double *rows, size_t cap, size_t *n);
@@ -401,6 +423,20 @@ int glu_gfix_server_create(glu_gfix_server **srv, size_t hbm_budget,
    (default 1).  Applies to requests admitted later. */
 int glu_gfix_server_configure(glu_gfix_server *srv, double switch_theta,
                               int use_graph, uint64_t restart_seed);
+/* Small lattices in shared launches (enable != 0; default off): FP64
+   Landau CG requests whose extents are all 2..16 and that ask for no
+   random transform skip the contexts.  When one is at the queue head,
+   every queued request of its shape, gf_tune, accuracy and max_iters is
+   fixed in one glu_gfix_facg_batch launch (one workgroup per lattice) on a
+   host thread of the server; the launch takes one slot and its scratch
+   (576 B per site per lattice) counts against the budget.  A lattice whose
+   FACG returns 123456789 keeps its input and goes back to the queue head
+   for Landau()'s restarts on a context, as glu_gfix_landau takes them.  A
+   request's result is then glu_gfix_facg_batch's for that lattice, which
+   does not depend on the other lattices of the launch.  Applies to
+   requests admitted later.  No counterpart in GLU, which fixes one
+   configuration per call (Wrappers/GF_wrap.c:179-212). */
+int glu_gfix_server_set_batching(glu_gfix_server *srv, int enable);
 int glu_gfix_server_submit(glu_gfix_server *srv, const size_t dims[4],
                            int gftype, int precision, int method,
                            double gf_tune, double accuracy, size_t max_iters,

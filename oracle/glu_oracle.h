@@ -109,6 +109,7 @@ double or_fa_deriv(const or_lat *L, const double *U, double *F); /* theta */
    sign -1 = FFTW_FORWARD, +1 = FFTW_BACKWARD, applied to `nb` arrays of
    prod(dims[0..nd-1]) complex entries laid out back to back */
 void   or_fft(double *data, const size_t *dims, int nd, int sign, size_t nb);
+void   or_set_fft_mode(int mode); /* 0 mixed radix (default), 1 direct per-axis sums */
 void   or_dft_naive(double *data, const size_t *dims, int nd, int sign);
 void   or_psq_table(const or_lat *L, int D, double *psq); /* MAX/p^2 */
 void   or_fourier_accelerate(const or_lat *L, double *F, const double *psq,

@@ -46,6 +46,14 @@ static void fft_rec(const cplx *in, size_t is, cplx *out, size_t N,
   }
 }
 
+/* 0: the mixed-radix decimation in time above; 1: the direct sum
+   out[k] = sum_n in[n] w[(n k) mod N], n ascending, along each axis --
+   another exact DFT with rounding of its own, to measure how far a
+   trajectory depends on the arithmetic of the transform (FFTW's codelets,
+   which GLU calls, are a third such arithmetic) */
+static int or_fft_mode = 0;
+void or_set_fft_mode(int mode) { or_fft_mode = mode; }
+
 /* in-place unnormalised multi-dimensional DFT over nd axes */
 void or_fft(double *data, const size_t *dims, int nd, int sign, size_t nb)
 {
@@ -71,7 +79,15 @@ void or_fft(double *data, const size_t *dims, int nd, int sign, size_t nb)
           const size_t lo = pc % stride, hi = pc / stride;
           cplx *base = D + b * vol + lo + hi * stride * N;
           for (size_t n = 0; n < N; n++) buf[n] = base[n * stride];
-          fft_rec(buf, 1, res, N, w, 1, N, tmp);
+          if (or_fft_mode == 1) {
+            for (size_t k = 0; k < N; k++) {
+              cplx acc = 0.0;
+              for (size_t n = 0; n < N; n++) acc += buf[n] * w[(n * k) % N];
+              res[k] = acc;
+            }
+          } else {
+            fft_rec(buf, 1, res, N, w, 1, N, tmp);
+          }
           for (size_t n = 0; n < N; n++) base[n * stride] = res[n];
         }
       free(buf);

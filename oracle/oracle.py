@@ -76,6 +76,7 @@ def lib():
             "or_latt_deriv": (C.c_double, [LP, DP, SZ, C.c_int, DP]),
             "or_fa_deriv": (C.c_double, [LP, DP, DP]),
             "or_fft": (None, [DP, C.POINTER(SZ), C.c_int, C.c_int, SZ]),
+            "or_set_fft_mode": (None, [C.c_int]),
             "or_dft_naive": (None, [DP, C.POINTER(SZ), C.c_int, C.c_int]),
             "or_psq_table": (None, [LP, C.c_int, DP]),
             "or_fourier_accelerate": (None, [LP, DP, DP, C.c_int, SZ]),
@@ -314,6 +315,20 @@ class planted:
 
     def __exit__(self, *a):
         lib().or_set_plant(0)
+
+
+class direct_dft:
+    """Context manager: the oracle's Fourier transforms as direct per-axis
+    sums instead of its mixed-radix decimation (or_set_fft_mode) -- an
+    exact DFT with other rounding, whose trajectories show how far the
+    transform's arithmetic moves them."""
+
+    def __enter__(self):
+        lib().or_set_fft_mode(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().or_set_fft_mode(0)
 
 
 def fft(data: np.ndarray, dims, nd: int, sign: int) -> np.ndarray:

@@ -103,3 +103,22 @@ def test_cuts_struct_refusals_need_no_device():
     assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=1, type=1, max_mom=3)) == -1
     assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=6, type=1, max_mom=3, definition=1)) == -1
     assert lib.cuts_struct(C.addressof(site), _CutInfo(dir=6, type=9, max_mom=3)) == -1
+
+
+@pytest.mark.parametrize("dims,nlat,word", [
+    ((8, 8, 8, 32), 1, "extent"),      # a pencil longer than the direct DFT holds
+    ((1, 8, 8, 8), 1, "extent"),
+    ((8, 8, 8, 8), 70000, "65535"),    # more lattices than one launch's grid
+])
+def test_facg_batch_refusals_need_no_device(dims, nlat, word):
+    """glu_gfix_facg_batch refuses shapes and batch sizes its one-workgroup
+    kernel cannot hold before any device work; an empty batch of a valid
+    shape is a no-op."""
+    lib = gl.lib()
+    it = (C.c_size_t * 1)()
+    th = (C.c_double * 1)()
+    d = (C.c_size_t * 4)(*dims)
+    rc = lib.glu_gfix_facg_batch(d, nlat, C.c_void_p(8), 0.08, 1e-14, 100, it, th, None, 0)
+    assert rc == -1 and word in lib.glu_gfix_last_error().decode()
+    ok = (C.c_size_t * 4)(8, 8, 8, 8)
+    assert lib.glu_gfix_facg_batch(ok, 0, C.c_void_p(8), 0.08, 1e-14, 100, it, th, None, 0) == 0

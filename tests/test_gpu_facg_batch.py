@@ -1,0 +1,187 @@
+"""GPU parity of glu_gfix_facg_batch -- FACG() of many small lattices of one
+shape in one launch, one workgroup per lattice -- against the CPU oracle's
+FACG on the same seeded inputs, through the C ABI.
+
+Every lattice of a batch is held to the single-lattice bar of the context
+path:
+  trajectory theta_k / alpha_k / beta_k, SD start + 20 iterations:
+      1e-12 relative (1e-11 after the first 10 iterations: ROW_TOL), or 20x
+      the oracle's own spread where that is larger:
+      over summation orders (1 virtual thread against 2, 3, 4 and 8) and
+      over the arithmetic of the transform (the oracle's mixed-radix DFT
+      against direct per-axis sums, the form the kernel computes);
+  final state: theta < accuracy, functional 1e-10 relative (where the two
+      oracle orders end on one Gribov copy), plaquette preserved to 1e-14
+      after GLU's reunitarisation;
+  GLU's continuation below 1e3 acc (with its stale reduction slots) and its
+      123456789 sentinel: the oracle's iteration count and every row.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+from gpu_common import check_trajectory, heatbath_config, rel
+
+pytestmark = pytest.mark.gpu
+
+gl = pytest.importorskip("glu_amd")
+torch = pytest.importorskip("torch")
+
+SENTINEL = 123456789
+# summation orders of the oracle whose spread, with the direct-DFT run
+# below, sets the trajectory allowance: on lattices of a few thousand
+# sites the parabola fit amplifies rounding by about 1e3 over 20
+# iterations, and two orders alone can leave a row of the GPU's order
+# (strided per-thread sums, then a tree) just outside
+ALT_THREADS = (2, 3, 4, 8)
+# base bar per row: 1e-12 for the SD start and the first 10 iterations,
+# 1e-11 after them.  The kernel's arithmetic differs from the oracle's in
+# every operation, not only in its sums: it contracts multiply-adds into
+# FMAs (the oracle is plain C with contraction off), takes its twiddles
+# from the device's cos/sin, and transforms by direct sums; the fit then
+# amplifies such differences about 1e3-fold by row 20, so that on a 6^4
+# lattice one row (15) of one input moved by 2.1e-12 where the oracle's own
+# orders agree to 1e-13
+ROW_TOL = np.where(np.arange(64) < 11, 1e-12, 1e-11)
+
+
+def copies(dims, n, seed0=100):
+    """n gauge copies of one heatbath config: random transforms drawn from
+    GLU's streams seeded seed0 + k."""
+    U, plaq = heatbath_config(dims)
+    L = O.Lattice(dims, nth=4)
+    out = []
+    for k in range(n):
+        W = U.copy()
+        W.setflags(write=True)
+        L.random_gtrans(W, seed=seed0 + k, nth=4)
+        out.append(W)
+    return out, plaq
+
+
+def run_batch(dims, lats, acc=1e-14, max_iters=1000, trace_cap=64):
+    """One launch over the lattices `lats` (host arrays); returns the
+    iteration counts, final thetas, trace rows and the fixed links."""
+    V = int(np.prod(dims))
+    host = np.ascontiguousarray(np.stack(lats)).reshape(len(lats), V, 4, 18)
+    dev = torch.from_numpy(host).cuda()
+    torch.cuda.synchronize()
+    print(f"[facg_batch] {dims} x {len(lats)} lattices, acc {acc}, max_iters {max_iters}", flush=True)
+    its, th, tr = gl.facg_batch(dims, dev.data_ptr(), len(lats), 0.08, acc, max_iters, trace_cap)
+    return its, th, tr, dev.cpu().numpy()
+
+
+def oracle_run(W, dims, nth, **kw):
+    L = O.Lattice(dims, nth=nth)
+    X = W.copy()
+    r = L.facg(X, trace=4000, **kw)
+    return L, X, r
+
+
+def oracle_alts(W, dims, **kw):
+    """The oracle in other summation orders, and with its transforms as
+    direct per-axis sums (the kernel's form of the DFT; the oracle's
+    default is a mixed-radix decimation, GLU's FFTW a third arithmetic):
+    their spread is the rounding the trajectory is entitled to."""
+    runs = [oracle_run(W, dims, nth, **kw) for nth in ALT_THREADS]
+    with O.direct_dft():
+        runs.append(oracle_run(W, dims, 1, **kw))
+    return runs
+
+
+@pytest.mark.parametrize("dims,n", [((8, 8, 8, 8), 4), ((6, 6, 6, 6), 3), ((4, 6, 8, 12), 2)])
+def test_batch_trajectories_and_final_state(dims, n):
+    lats, plaq = copies(dims, n)
+    its, th, tr, out = run_batch(dims, lats)
+    for k, W in enumerate(lats):
+        L, X1, r1 = oracle_run(W, dims, 1)
+        runs = oracle_alts(W, dims)
+        alts = [r for _, _, r in runs]
+        rows = tr[k][: min(64, len(r1["trace_theta"]))]
+        check_trajectory(f"facg_batch {dims} lattice {k}", rows, r1, alts, n=21, base_tol=ROW_TOL)
+        assert its[k] != SENTINEL and th[k] < 1e-14
+        # iteration count: as close to the oracle's as its own orders are to each other
+        spread = max(abs(r["iters"] - r1["iters"]) for r in alts)
+        assert abs(its[k] - r1["iters"]) <= max(2, r1["iters"] // 20, 2 * spread), \
+            (its[k], r1["iters"], [r["iters"] for r in alts])
+        G = out[k].reshape(-1, 4, 18).copy()
+        L.reunit(G)  # as GLU's Landau() does before its report
+        assert abs(L.plaquette(G) - plaq) < 1e-14
+        f_o = [L.functional(X1)] + [L.functional(X) for _, X, _ in runs]
+        if max(rel(f, f_o[0]) for f in f_o) < 1e-10:  # the oracle's orders end on one Gribov copy
+            assert rel(L.functional(G), f_o[0]) < 1e-10
+
+
+def test_batch_matches_the_context_path():
+    """Each lattice of a batch against the one-lattice context path
+    (glu_gfix_run, FACG): the same Gribov copy and iteration count within
+    the rounding of two summation orders."""
+    dims = (8, 8, 8, 8)
+    lats, _ = copies(dims, 3, seed0=300)
+    its, th, _, out = run_batch(dims, lats, trace_cap=1)
+    L = O.Lattice(dims, nth=4)
+    for k, W in enumerate(lats):
+        with gl.Gfix(dims, gl.LANDAU, gl.FP64) as g:
+            g.set_links(W)
+            it_c, th_c = g.run(gl.CG, 0.08, 1e-14, 1000)
+            Wc = g.get_links()
+        assert th_c < 1e-14 and th[k] < 1e-14
+        assert abs(its[k] - it_c) <= max(2, it_c // 20), (its[k], it_c)
+        assert rel(L.functional(out[k].reshape(-1, 4, 18)), L.functional(Wc)) < 1e-10
+
+
+def test_continuation_with_stale_slots_and_sentinel():
+    """GLU's continuation: the first segment runs out of iterations with
+    theta below 1e3 acc and restarts from a steepest-descent step that reads
+    the previous segment's reduction slots; every row of both segments and
+    the iteration count against the oracle.  A lattice that cannot reach
+    the accuracy in its iterations returns the sentinel, and a lattice that
+    is already fixed (unit links) returns 0 at once, in the same launch."""
+    dims = (8, 8, 8, 8)
+    lats, _ = copies(dims, 1, seed0=500)
+    W = lats[0]
+    _, _, rfull = oracle_run(W, dims, 4)
+    th_rows = rfull["trace_theta"][1:]
+    k = int(np.argmax(th_rows < 1e-12)) + 1  # first segment ends with theta < 1e3 acc
+    _, X1, r1 = oracle_run(W, dims, 1, max_iters=k)
+    r8 = [r for _, _, r in oracle_alts(W, dims, max_iters=k)]
+    assert r1["iters"] != SENTINEL and len(r1["trace_theta"]) > k + 1  # a continuation ran
+    unit = np.zeros_like(W)
+    unit[:, :, [0, 8, 16]] = 1.0
+    n = len(r1["trace_theta"])
+    its, th, tr, out = run_batch(dims, [W, W.copy(), unit], max_iters=k, trace_cap=n + 8)
+    for j in (0, 1):
+        check_trajectory(f"facg_batch continuation lattice {j}", tr[j][:n], r1, r8, n=n,
+                         base_tol=np.where(np.arange(n) < 11, 1e-12, 1e-11))
+        assert its[j] == r1["iters"] and th[j] < 1e-14
+        assert np.abs(out[j].reshape(-1, 4, 18) - X1).max() < 1e-9
+    # the two copies of one input in one launch: bitwise the same
+    assert np.array_equal(out[0], out[1]) and its[0] == its[1]
+    assert its[2] == 0 and th[2] == 0.0 and np.array_equal(out[2].reshape(unit.shape), unit)
+    # unreachable accuracy in 5 iterations: GLU's sentinel, the rows of the oracle
+    _, _, rs = oracle_run(W, dims, 1, max_iters=5)
+    rs8 = [r for _, _, r in oracle_alts(W, dims, max_iters=5)]
+    its, th, tr, _ = run_batch(dims, [W], max_iters=5, trace_cap=8)
+    assert rs["iters"] == SENTINEL and its[0] == SENTINEL
+    check_trajectory("facg_batch sentinel", tr[0][:6], rs, rs8, n=6)
+
+
+
+@pytest.mark.parametrize("wg", ["128", "256"])
+def test_other_workgroup_sizes(wg, monkeypatch):
+    """The 128- and 256-thread forms of the kernel (GLU_GFIX_BATCH_WG)
+    against the oracle: only the summation order changes."""
+    dims = (8, 8, 8, 8)
+    monkeypatch.setenv("GLU_GFIX_BATCH_WG", wg)
+    lats, plaq = copies(dims, 2, seed0=700)
+    its, th, tr, out = run_batch(dims, lats)
+    for k, W in enumerate(lats):
+        L, X1, r1 = oracle_run(W, dims, 1)
+        alts = [r for _, _, r in oracle_alts(W, dims)]
+        check_trajectory(f"facg_batch wg {wg} lattice {k}", tr[k][: len(r1["trace_theta"])], r1, alts, n=21,
+                         base_tol=ROW_TOL)
+        assert its[k] != SENTINEL and th[k] < 1e-14
+        G = out[k].reshape(-1, 4, 18).copy()
+        L.reunit(G)
+        assert abs(L.plaquette(G) - plaq) < 1e-14

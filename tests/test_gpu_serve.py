@@ -264,3 +264,86 @@ def test_coulomb_request_does_not_hold_up_landau_runs():
     assert (rc[0], rc[1]) == (Wc[1], Wc[2]) and np.array_equal(bc.cpu().numpy(), Wc[0])
     for r, b in zip(rl, bl):
         assert (r[0], r[1]) == (Wl[1], Wl[2]) and np.array_equal(b.cpu().numpy(), Wl[0])
+
+
+def batch_alone(dims, U, max_iters=1000):
+    """glu_gfix_facg_batch on this lattice alone: (links, iterations, theta)."""
+    b = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+    torch.cuda.synchronize()
+    its, th = gl.facg_batch(dims, b.data_ptr(), 1, 0.08, 1e-14, max_iters)
+    return b.cpu().numpy(), its[0], float(th[0])
+
+
+BATCH_REQUESTS = [  # dims, precision, input seed: the FP64 8^4 and 6^4 requests share launches
+    ((8, 8, 8, 8), gl.FP64, 11),
+    ((8, 8, 8, 32), gl.FP64, 12),   # an extent above 16: a context
+    ((8, 8, 8, 8), gl.FP64, 13),
+    ((6, 6, 6, 6), gl.FP64, 14),
+    ((8, 8, 8, 8), gl.MIXED, 15),   # mixed precision: a context
+    ((8, 8, 8, 8), gl.FP64, 16),
+    ((6, 6, 6, 6), gl.FP64, 17),
+    ((8, 8, 8, 8), gl.FP64, 18),
+]
+
+
+def test_small_lattices_share_launches():
+    """With batching on, the FP64 Landau requests of 8^4 and 6^4 lattices
+    are fixed in shared launches: each result equals, bitwise, the one
+    glu_gfix_facg_batch gives for that lattice alone, while the others
+    are served on contexts and equal their stand-alone runs."""
+    Us = [heatbath_config(d, sweeps=20, seed=s, gseed=s + 100)[0] for d, _, s in BATCH_REQUESTS]
+    ref = []
+    for (dims, prec, _), U in zip(BATCH_REQUESTS, Us):
+        small = prec == gl.FP64 and max(dims) <= 16
+        ref.append(batch_alone(dims, U) if small else alone(dims, U, precision=prec, landau=True))
+    bufs = [torch.from_numpy(np.ascontiguousarray(U)).cuda() for U in Us]
+    torch.cuda.synchronize()
+    with gl.Server(max_active=3, batching=True) as srv:
+        tickets = [srv.submit(d, b.data_ptr(), gl.LANDAU, p) for (d, p, _), b in zip(BATCH_REQUESTS, bufs)]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == len(BATCH_REQUESTS)
+    for k, ((W, it, th), (it2, th2, lat), b) in enumerate(zip(ref, res, bufs)):
+        assert th < 1e-14 and (it2, th2) == (it, th), (k, it, it2, th, th2)
+        assert np.array_equal(b.cpu().numpy(), W), k
+
+
+def test_batched_request_restarts_on_a_context():
+    """A small-lattice request whose FACG fails in the shared launch keeps
+    its input and takes Landau()'s restarts on a context: the iteration
+    count, theta and links of the policy replayed by hand (attempt k on the
+    input under the k-th random_transform from GLU's streams seeded
+    SEED = restart_seed)."""
+    dims = (8, 8, 8, 8)
+    U, _ = heatbath_config(dims, sweeps=20)
+    seed = 5
+    _, full, _ = batch_alone(dims, U)
+    found = False
+    with gl.Gfix(dims, gl.LANDAU, gl.FP64) as g:
+        g.set_links(U)
+        g.snapshot()
+        for M in range(full // 3, full, max(1, full // 12)):
+            _, it0, _ = batch_alone(dims, U, M)
+            if it0 != 123456789:
+                continue
+            outcome = None
+            g.rng_seed(seed, 256)
+            for k in range(1, 7):
+                g.restore()
+                g.glu_random_transform()
+                loc, th = g.run(gl.CG, 0.08, 1e-14, M)
+                if loc != 123456789:
+                    outcome = (k, loc, th, g.get_links())
+                    break
+            if outcome is None:
+                continue
+            k, loc, th, W = outcome
+            b = torch.from_numpy(np.ascontiguousarray(U)).cuda()
+            torch.cuda.synchronize()
+            with gl.Server(max_active=2, batching=True, restart_seed=seed) as srv:
+                it, th2, _ = srv.wait(srv.submit(dims, b.data_ptr(), gl.LANDAU, gl.FP64, max_iters=M))
+            assert it == 123456789 + (k - 1) * M + loc and th2 == th
+            assert np.array_equal(b.cpu().numpy(), W)
+            found = True
+            break
+    assert found, "no max_iters gave a failure in the shared launch followed by a successful restart"

@@ -178,6 +178,26 @@ def test_fft_vs_definition(dims):
         assert np.abs(f - ref).max() < 1e-12 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("dims", [(2, 3, 4, 6), (8, 8, 2, 4), (6, 6, 6, 6)])
+def test_direct_dft_mode_vs_definition(dims):
+    """The oracle's direct per-axis mode (the rounding reference for a
+    transform computed as plain sums) is the same DFT: against the
+    definition, numpy and the mixed-radix mode, and it really is another
+    arithmetic (not bitwise the mixed-radix result)."""
+    rng = np.random.default_rng(5)
+    V = int(np.prod(dims))
+    a = rng.standard_normal(V) + 1j * rng.standard_normal(V)
+    for sign in (-1, 1):
+        f = O.fft(a[None], dims, 4, sign)[0]
+        with O.direct_dft():
+            g = O.fft(a[None], dims, 4, sign)[0]
+        n = O.dft_naive(a, dims, 4, sign)
+        assert np.abs(g - n).max() < 1e-12 * np.abs(n).max()
+        assert np.abs(g - f).max() < 1e-12 * np.abs(f).max()
+    assert not np.array_equal(f, g)
+    assert np.array_equal(O.fft(a[None], dims, 4, 1)[0], f)  # the mode is back to mixed radix
+
+
 def test_fft_closed_forms():
     dims = (4, 6, 8, 3)
     V = int(np.prod(dims))
