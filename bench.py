@@ -463,12 +463,30 @@ def batch_main(a):
         "speedup_vs_context_path": round((B * a.steps / total) / (nctx / t_ctx), 2),
         "server": served,
         "stream_GBps": round(gbs, 1) if gbs else None,
-        "roofline": {"kernel": "k_facg_batch (whole FACG per workgroup)", "bound": "hbm",
-                     "achieved": round(gbs, 1) if gbs else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(gbs / PEAK_HBM_GBS, 4) if gbs else None, "traffic": None},
+        "roofline": batch_roofline(gbs, launch_s, dims, B, sum(its_all) / a.steps),
         "cpu_baseline": cpu,
     }
     emit(line)
+
+
+def batch_roofline(gbs, launch_s, dims, B, lattice_iters):
+    """Roofline of the shared launch: the logical bytes of the FACG
+    iterations it ran (3072 B per site per iteration) over its time, and
+    beside them the HBM bytes the counters measured for the same launch
+    (profiles/traffic.json, FETCH_SIZE x 2 + WRITE_SIZE as calibrated on a
+    copy kernel of known size)."""
+    out = {"kernel": "k_facg_batch (whole FACG per workgroup)", "bound": "hbm",
+           "achieved": round(gbs, 1) if gbs else None, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": round(gbs / PEAK_HBM_GBS, 4) if gbs else None, "traffic": None}
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    key = f"facg_batch_fp64_{'x'.join(map(str, dims))}_{B}"
+    if os.path.exists(tfile) and gbs:
+        t = json.load(open(tfile)).get(key)
+        if t:
+            out["traffic"] = t
+            out["frac_counter"] = round(t / launch_s / 1e9 / PEAK_HBM_GBS, 4)
+            out["traffic_per_site_iteration"] = round(t / (lattice_iters * math.prod(dims)), 1)
+    return out
 
 
 def batch_cpu_baseline(src, dims, B):

@@ -17,7 +17,8 @@
 // With glu_gfix_server_set_batching, FP64 Landau CG requests of small
 // lattices (every extent 2..16) skip the contexts: every queued request of
 // one shape and settings is fixed in one shared launch, one workgroup per
-// lattice (facg_batch.hip), on a host thread of its own; a lattice whose
+// lattice (facg_batch.hip), on a host thread of its own, and requests of
+// that key arriving meanwhile gather for the next launch; a lattice whose
 // FACG fails goes back to the queue head and takes Landau()'s restarts on
 // a context.
 //
@@ -331,22 +332,36 @@ void worker_main(glu_gfix_server *s) {
     }
   };
   for (;;) {
-    // small lattices: when the queue head can share a launch, every queued
-    // request of its shape and settings joins it, as many as the memory
-    // budget holds; a launch takes one slot
+    // small lattices: the first queued request that can share a launch,
+    // with every queued request of its shape and settings, as many as the
+    // memory budget holds.  A launch takes one slot; while one runs for a
+    // shape and settings, the requests of that key gather for the next
     std::vector<Batch> launched;
-    while (!s->queue.empty() && (int)(active.size() + batches.size() + launched.size()) < s->max_active) {
-      const Request &h = s->reqs[s->queue.front()];
-      if (!batchable(s, h)) break;
-      const size_t hd[4] = {h.dims[0], h.dims[1], h.dims[2], h.dims[3]};
-      const double gf = h.gf, acc = h.acc;
-      const size_t mi = h.max_iters, per = hd[0] * hd[1] * hd[2] * hd[3] * kBatchScratchPerSite;
+    auto same_key = [](const Request &a, const Request &b) {
+      return same_shape(a.dims, b.dims) && a.gf == b.gf && a.acc == b.acc && a.max_iters == b.max_iters;
+    };
+    auto key_running = [&](const Request &h) {
+      for (const Batch &b : batches)
+        if (same_key(*b.reqs[0], h)) return true;
+      for (const Batch &b : launched)
+        if (same_key(*b.reqs[0], h)) return true;
+      return false;
+    };
+    for (size_t qi = 0; qi < s->queue.size() && (int)(active.size() + batches.size() + launched.size()) <
+                                                      s->max_active;) {
+      const Request &h = s->reqs[s->queue[qi]];
+      if (!batchable(s, h) || key_running(h)) {
+        ++qi;
+        continue;
+      }
+      const size_t per = h.dims[0] * h.dims[1] * h.dims[2] * h.dims[3] * kBatchScratchPerSite;
       const size_t room = std::min(s->budget > s->used ? (s->budget - s->used) / per : 0, kBatchMaxLattices);
       if (room == 0) break;  // waits for memory
+      const Request key = h;
       Batch b;
       for (auto it = s->queue.begin(); it != s->queue.end() && b.reqs.size() < room;) {
         Request &r = s->reqs[*it];
-        if (batchable(s, r) && same_shape(r.dims, hd) && r.gf == gf && r.acc == acc && r.max_iters == mi) {
+        if (batchable(s, r) && same_key(r, key)) {
           settle(r);
           b.reqs.push_back(&r);
           it = s->queue.erase(it);
@@ -360,16 +375,20 @@ void worker_main(glu_gfix_server *s) {
       b.running = std::make_shared<std::atomic<int>>(1);
       launched.push_back(std::move(b));
     }
-    // admission: FIFO, as many as the budget and the slot count allow
+    // admission of the others: FIFO among themselves, as many as the budget
+    // and the slot count allow
     std::vector<Request *> admitted;
-    while (!s->queue.empty() && (int)(active.size() + batches.size() + launched.size() + admitted.size()) <
-                                    s->max_active) {
-      Request &r = s->reqs[s->queue.front()];
-      if (batchable(s, r)) break;  // waits for room for a shared launch
+    for (size_t qi = 0; qi < s->queue.size() &&
+                        (int)(active.size() + batches.size() + launched.size() + admitted.size()) < s->max_active;) {
+      Request &r = s->reqs[s->queue[qi]];
+      if (batchable(s, r)) {  // gathers for a shared launch
+        ++qi;
+        continue;
+      }
       int err = 0;
       glu_gfix_ctx *c = acquire(s, r, &err);
       if (err) {  // creation failed: fail the request, keep serving
-        s->queue.pop_front();
+        s->queue.erase(s->queue.begin() + qi);
         r.t_done = Clock::now();
         r.state = RQ_FAILED;
         r.rc = err;
@@ -381,7 +400,7 @@ void worker_main(glu_gfix_server *s) {
       r.ctx = c;
       settle(r);
       admitted.push_back(&r);
-      s->queue.pop_front();
+      s->queue.erase(s->queue.begin() + qi);
     }
     if (active.empty() && admitted.empty() && batches.empty() && launched.empty()) {
       if (s->stop && s->queue.empty()) break;

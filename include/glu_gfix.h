@@ -425,11 +425,13 @@ int glu_gfix_server_configure(glu_gfix_server *srv, double switch_theta,
                               int use_graph, uint64_t restart_seed);
 /* Small lattices in shared launches (enable != 0; default off): FP64
    Landau CG requests whose extents are all 2..16 and that ask for no
-   random transform skip the contexts.  When one is at the queue head,
-   every queued request of its shape, gf_tune, accuracy and max_iters is
-   fixed in one glu_gfix_facg_batch launch (one workgroup per lattice) on a
-   host thread of the server; the launch takes one slot and its scratch
-   (576 B per site per lattice) counts against the budget.  A lattice whose
+   random transform skip the contexts.  Every queued request of one shape,
+   gf_tune, accuracy and max_iters is fixed in one glu_gfix_facg_batch
+   launch (one workgroup per lattice) on a host thread of the server; the
+   launch takes one slot and its scratch (576 B per site per lattice)
+   counts against the budget, and while it runs, later requests of that
+   key gather for the next.  The other requests are admitted FIFO among
+   themselves past the ones gathering.  A lattice whose
    FACG returns 123456789 keeps its input and goes back to the queue head
    for Landau()'s restarts on a context, as glu_gfix_landau takes them.  A
    request's result is then glu_gfix_facg_batch's for that lattice, which
