@@ -150,9 +150,117 @@ template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict_
   __syncthreads();
 }
 
+// The x, y and z transforms of one time slice of one field (a slab of
+// Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
+// kStage complex numbers hold: the slabs cross HBM once for the three axes
+// instead of three times.  Each axis is the same direct sum as bdft_axis,
+// in the same order, from one LDS buffer into the other.
+constexpr int kStage = 2048;
+
+template <int WG>
+__device__ __forceinline__ void lds_dft_axis(const double2 *__restrict__ in, double2 *__restrict__ out, int npts,
+                                             int s3, int N, int S, int sign, const double2 *tw) {
+  for (int p = threadIdx.x; p < npts; p += WG) {
+    const int k = ((p % s3) / S) % N;  // this point's index along the axis
+    const int base = p - k * S;        // the pencil's first point
+    cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll 4
+    for (int j = 0; j < N; j++) {
+      {
+        const double2 w = tw[(j * k) % N];
+        const double2 a = in[base + j * S];
+        acc = cadd(acc, cmul(mk<double>(a.x, a.y), mk<double>(w.x, sign < 0 ? w.y : -w.y)));
+      }
+    }
+    out[p] = make_double2(acc.re, acc.im);
+  }
+  __syncthreads();
+}
+
+// x, y, z transforms of the 4 T slabs of the 4 fields (requires
+// Lx Ly Lz <= kStage); lds holds two buffers of kStage
+template <int WG>
+__device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g, int sign,
+                                         const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
+  const int s3 = (int)g.S[3], per = kStage / s3, nslab = 4 * g.L[3];
+  double2 *b0 = lds, *b1 = lds + kStage;
+  for (int s0 = 0; s0 < nslab; s0 += per) {
+    const int npts = min(per, nslab - s0) * s3;
+    double2 *src = F + (long long)s0 * s3;  // slab (f, t) starts at (f T + t) s3
+    for (int p = threadIdx.x; p < npts; p += WG) b0[p] = src[p];
+    __syncthreads();
+    lds_dft_axis<WG>(b0, b1, npts, s3, g.L[0], 1, sign, tw[0]);
+    lds_dft_axis<WG>(b1, b0, npts, s3, g.L[1], (int)g.S[1], sign, tw[1]);
+    lds_dft_axis<WG>(b0, b1, npts, s3, g.L[2], (int)g.S[2], sign, tw[2]);
+    for (int p = threadIdx.x; p < npts; p += WG) src[p] = b1[p];
+    // the next round's loads write b0, which no thread reads after the
+    // barrier that ended the z sums
+  }
+  __syncthreads();
+}
+
+// along t, on pencils in registers: forward sums, x MAX / p^2, backward
+// sums -- the middle of the Fourier acceleration in one pass.  The scaled
+// forward values go back to the pencil's own sites and are read again for
+// the backward sums (the thread's own stores, still in cache), so that
+// every register array is indexed by constants.
+template <int WG>
+__device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo &g, const double2 (*tw)[kMaxExtent]) {
+  const int N = g.L[3];
+  const long long S = g.S[3], V = g.V;
+  const double mx = 4.0 * 4 / (double)V;
+  for (long long p = threadIdx.x; p < 4 * S; p += WG) {
+    const long long f = p / S, q = p % S;
+    const long long base = f * V + q;
+    int x[4];
+    bcoords(g, q, x);
+    const double kxyz = tw[0][x[0]].x + tw[1][x[1]].x + tw[2][x[2]].x;
+    cx<double> a[kMaxExtent];
+#pragma unroll
+    for (int j = 0; j < kMaxExtent; j++)
+      if (j < N) a[j] = ld<double>(F + base + j * S);
+    for (int k = 0; k < N; k++) {
+      cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < kMaxExtent; j++) {
+        if (j < N) {
+          const double2 w = tw[3][(j * k) % N];
+          acc = cadd(acc, cmul(a[j], mk<double>(w.x, w.y)));
+        }
+      }
+      const double kc = kxyz + tw[3][k].x;  // sum of cos(2 pi n_mu / L_mu), mu in order
+      const double fac = mx / ((kc == 4.0) ? 1.0 : 2.0 * (4.0 - kc));
+      st<double>(F + base + k * S, mk<double>(acc.re * fac, acc.im * fac));
+    }
+#pragma unroll
+    for (int j = 0; j < kMaxExtent; j++)
+      if (j < N) a[j] = ld<double>(F + base + j * S);
+    for (int k = 0; k < N; k++) {
+      cx<double> acc = mk<double>(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < kMaxExtent; j++) {
+        if (j < N) {
+          const double2 w = tw[3][(j * k) % N];
+          acc = cadd(acc, cmul(a[j], mk<double>(w.x, -w.y)));
+        }
+      }
+      st<double>(F + base + k * S, acc);
+    }
+  }
+  __syncthreads();
+}
+
 // forward transform, x MAX / p^2 (p^2(0) := 1, MAX = 4 ND / V), backward
 // transform: Gfix/FACG.c:77-95 with Gfix/Landau.c:275-284
-template <int WG> __device__ __forceinline__ void bfourier_accelerate(double2 *__restrict__ F, const BGeo &g, const double2 (*tw)[kMaxExtent]) {
+template <int WG>
+__device__ __forceinline__ void bfourier_accelerate(double2 *__restrict__ F, const BGeo &g,
+                                                    const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
+  if (g.S[3] <= kStage) {  // three passes over the fields
+    bdft_xyz<WG>(F, g, -1, tw, lds);
+    bdft_t_scale<WG>(F, g, tw);
+    bdft_xyz<WG>(F, g, +1, tw, lds);
+    return;
+  }
   for (int mu = 0; mu < 4; mu++) bdft_axis<WG>(F, g, mu, -1, tw[mu]);
   const double mx = 4.0 * 4 / (double)g.V;
   for (long long i = threadIdx.x; i < g.V; i += WG) {
@@ -264,6 +372,7 @@ __global__ __launch_bounds__(WG, 2) void k_facg_batch(double2 *const *__restrict
                                                           int keep_failed) {
   __shared__ double sh[3][kMaxWgWaves];
   __shared__ double2 tw[4][kMaxExtent];
+  __shared__ double2 stage[2 * kStage];
   const int b = blockIdx.x;
   if (b >= nlat) return;
   const long long V = g.V;
@@ -300,7 +409,7 @@ __global__ __launch_bounds__(WG, 2) void k_facg_batch(double2 *const *__restrict
   for (int seg = 0; seg < kMaxSegments; seg++) {
     // steepest-descent start
     theta = bderiv<WG>(U, in, g, sh) * norm + stale_theta;
-    bfourier_accelerate<WG>(in, g, tw);
+    bfourier_accelerate<WG>(in, g, tw, stage);
     double pr[3];
     bprobes<WG>(U, in, g, pr, sh);
     double val[3];
@@ -318,7 +427,7 @@ __global__ __launch_bounds__(WG, 2) void k_facg_batch(double2 *const *__restrict
     int did_ls = 0;
     for (;;) {
       theta = bderiv<WG>(U, in, g, sh) * norm;
-      bfourier_accelerate<WG>(in, g, tw);
+      bfourier_accelerate<WG>(in, g, tw, stage);
       bpr_sums<WG>(in, old, g, s2, sh);
       insum = s2[0];
       double beta = s2[1] / inold;
