@@ -191,10 +191,9 @@ int glu_gfix_luxury(glu_gfix_ctx *ctx, int method, double gf_tune,
    bound them).  Landau()'s random restarts are left to the caller
    (glu_gfix_landau on the failed ones).  Waits for the device's pending
    work, runs on a stream of its own and returns when every lattice has
-   stopped; allocates 192 B per site per
-   lattice of device scratch for the call.  For lattices whose iterations
-   are bound by launch latency (8^4: one iteration of the context path
-   takes about 75 us). */
+   stopped; allocates 576 B per site per lattice of device scratch for the
+   call.  For lattices whose iterations are bound by launch latency (8^4:
+   one iteration of the context path takes about 75 us). */
 int glu_gfix_facg_batch(const size_t dims[4], size_t nlat, double *links_dev, double gf_tune,
                         double accuracy, size_t max_iters, size_t *iters, double *theta,
                         double *trace, size_t trace_cap);
