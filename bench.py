@@ -572,6 +572,15 @@ def coulomb_main(a):
                 "frac": round(d["GBps"] / PEAK_HBM_GBS, 4), "traffic": None,
                 "achieved_stored": d["GBps_stored"], "frac_stored": round(d["GBps_stored"] / PEAK_HBM_GBS, 4),
                 "slice_launches": slaunch, "total_ms": d["total_ms"]}
+        # HBM bytes the counters measured per slice-site of a launch
+        # (profiles/traffic.json; a launch covers the slices still running)
+        tfile = os.path.join(ROOT, "profiles", "traffic.json")
+        per = (json.load(open(tfile)).get(f"c_deriv_fp64_{'x'.join(map(str, dims))}_per_slice_site")
+               if os.path.exists(tfile) else None)
+        if per:
+            roof["traffic"] = round(per * Lcu * slaunch / d["launches"], -5)
+            roof["traffic_per_slice_site"] = per
+            roof["frac_counter"] = round(per * Lcu * slaunch / (d["total_ms"] * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
     cpu = None if a.no_cpu else coulomb_cpu_baseline(g, dims)
     line = {
         "metric": "Coulomb configs/s to theta<1e-14 at 48^3x96 (96 slices batched)",
