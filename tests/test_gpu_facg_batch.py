@@ -90,7 +90,9 @@ def oracle_alts(W, dims, **kw):
     return runs
 
 
-@pytest.mark.parametrize("dims,n", [((8, 8, 8, 8), 4), ((6, 6, 6, 6), 3), ((4, 6, 8, 12), 2)])
+@pytest.mark.parametrize("dims,n", [((8, 8, 8, 8), 4), ((6, 6, 6, 6), 3), ((4, 6, 8, 12), 2),
+                                    ((5, 7, 6, 9), 2),      # odd extents
+                                    ((16, 12, 12, 4), 2)])  # time slices of 2304 sites: nine transform passes
 def test_batch_trajectories_and_final_state(dims, n):
     lats, plaq = copies(dims, n)
     its, th, tr, out = run_batch(dims, lats)
