@@ -259,7 +259,8 @@ def main():
         cpu = cpu_baseline(gm, dims, sorted(ref_its)[len(ref_its) // 2], a.cpu_iters)
 
     line = {
-        "metric": METRIC, "value": round(mixed_value, 4), "unit": "configs/s", "n_gpus": 1,
+        "metric": METRIC if dims == (32, 32, 32, 64) else METRIC.replace("32^3x64", "x".join(map(str, dims))),
+        "value": round(mixed_value, 4), "unit": "configs/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * total / a.steps, 2),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64",
         "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, "

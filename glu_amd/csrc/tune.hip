@@ -109,6 +109,10 @@ template <typename T> Tuning tune(glu_gfix_ctx *c) {
       launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, nullptr, c->stream, false, 0)) {
     float best = 1e30f;
     for (int v = 0; v < 3; v++) {
+      // a variant whose x values do not divide Lx has no kernel: the pass
+      // would fall back to single-axis passes, so it is not a candidate
+      if (!launch_fft_zt<T>(F, V, 4, c->ax[0], c->ax[1], c->ax[2], c->ax[3], c->ps, nullptr, c->stream, false, v))
+        continue;
       const float ms = time_ms(
           c,
           [&] {
