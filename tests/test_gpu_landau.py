@@ -312,3 +312,18 @@ def test_autotune_changes_speed_only():
         assert abs(g.measure()["plaquette"] - plaq) < 1e-14
     with fresh(dims, U) as g:
         assert g.autotune().startswith("(cached)")
+
+
+def test_autotune_times_only_z_t_variants_that_exist():
+    """At Lx = 8 the z-t variant with 16 x values per workgroup has no
+    kernel; the tuner leaves it out instead of timing an empty launch (and
+    then sending every Fourier acceleration to single-axis passes)."""
+    dims = (8, 8, 8, 8)
+    U, _ = heatbath_config(dims, sweeps=10)
+    with fresh(dims, U, gl.MIXED) as g:
+        _, F = g.stage_deriv()
+        before = g.stage_fa(F)
+        rep = g.autotune()
+        assert "z-t column pass variant 0" in rep and "z-t column pass variant 2" not in rep, rep
+        after = g.stage_fa(F)
+        assert np.abs(after - before).max() < 1e-14 * np.abs(before).max()
