@@ -152,10 +152,12 @@ template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict_
 
 // The x, y and z transforms of one time slice of one field (a slab of
 // Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
-// kStage complex numbers hold: the slabs cross HBM once for the three axes
+// stage_cx<WG>() complex numbers hold: the slabs cross HBM once for the three axes
 // instead of three times.  Each axis is the same direct sum as bdft_axis,
 // in the same order, from one LDS buffer into the other.
-constexpr int kStage = 2048;
+// 4096 (two 64 KiB buffers) with 512 threads, which hold a CU alone;
+// 2048 with fewer, whose workgroups share a CU
+template <int WG> constexpr int stage_cx() { return WG >= 512 ? 4096 : 2048; }
 
 template <int WG>
 __device__ __forceinline__ void lds_dft_axis(const double2 *__restrict__ in, double2 *__restrict__ out, int npts,
@@ -178,10 +180,11 @@ __device__ __forceinline__ void lds_dft_axis(const double2 *__restrict__ in, dou
 }
 
 // x, y, z transforms of the 4 T slabs of the 4 fields (requires
-// Lx Ly Lz <= kStage); lds holds two buffers of kStage
+// Lx Ly Lz <= stage_cx); lds holds two buffers of stage_cx
 template <int WG>
 __device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g, int sign,
                                          const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
+  constexpr int kStage = stage_cx<WG>();
   const int s3 = (int)g.S[3], per = kStage / s3, nslab = 4 * g.L[3];
   double2 *b0 = lds, *b1 = lds + kStage;
   for (int s0 = 0; s0 < nslab; s0 += per) {
@@ -255,7 +258,7 @@ __device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo
 template <int WG>
 __device__ __forceinline__ void bfourier_accelerate(double2 *__restrict__ F, const BGeo &g,
                                                     const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
-  if (g.S[3] <= kStage) {  // three passes over the fields
+  if (g.S[3] <= stage_cx<WG>()) {  // three passes over the fields
     bdft_xyz<WG>(F, g, -1, tw, lds);
     bdft_t_scale<WG>(F, g, tw);
     bdft_xyz<WG>(F, g, +1, tw, lds);
@@ -372,7 +375,7 @@ __global__ __launch_bounds__(WG, 2) void k_facg_batch(double2 *const *__restrict
                                                           int keep_failed) {
   __shared__ double sh[3][kMaxWgWaves];
   __shared__ double2 tw[4][kMaxExtent];
-  __shared__ double2 stage[2 * kStage];
+  __shared__ double2 stage[2 * stage_cx<WG>()];
   const int b = blockIdx.x;
   if (b >= nlat) return;
   const long long V = g.V;

@@ -92,7 +92,7 @@ def oracle_alts(W, dims, **kw):
 
 @pytest.mark.parametrize("dims,n", [((8, 8, 8, 8), 4), ((6, 6, 6, 6), 3), ((4, 6, 8, 12), 2),
                                     ((5, 7, 6, 9), 2),      # odd extents
-                                    ((16, 12, 12, 4), 2)])  # time slices of 2304 sites: nine transform passes
+                                    ((16, 12, 12, 4), 2)])  # time slices of 2304 sites
 def test_batch_trajectories_and_final_state(dims, n):
     lats, plaq = copies(dims, n)
     its, th, tr, out = run_batch(dims, lats)
@@ -173,8 +173,9 @@ def test_continuation_with_stale_slots_and_sentinel():
 @pytest.mark.parametrize("wg", ["128", "256"])
 def test_other_workgroup_sizes(wg, monkeypatch):
     """The 128- and 256-thread forms of the kernel (GLU_GFIX_BATCH_WG)
-    against the oracle: only the summation order changes."""
-    dims = (8, 8, 8, 8)
+    against the oracle: only the summation order changes.  Their LDS stage
+    holds 2048 sites, so time slices of 2304 take the nine-pass form."""
+    dims = (16, 12, 12, 4)
     monkeypatch.setenv("GLU_GFIX_BATCH_WG", wg)
     lats, plaq = copies(dims, 2, seed0=700)
     its, th, tr, out = run_batch(dims, lats)
