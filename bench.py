@@ -201,6 +201,10 @@ def main():
     mixed_value = a.steps / total
     print(f"[bench] mixed: {mixed_value:.3f} configs/s, iters {its} (fp32 {n32} + fp64 {n64} in all), "
           f"restarts {restarts}", file=sys.stderr, flush=True)
+    host = host_boundary(gm, a.seed + 1)
+    # the timed mean solve with one upload and one download added
+    host["configs_per_s_with_transfers"] = round(
+        1.0 / (total / a.steps + 1e-3 * (host["upload_ms"] + host["download_ms"])), 4)
 
     fp64 = None
     if not a.no_fp64:
@@ -282,6 +286,7 @@ def main():
         "stream_GBps": round(gbs, 1),
         "fp64": fp64,
         "mixed_speedup_vs_fp64": round(mixed_value / fp64["configs_per_s"], 3) if fp64 else None,
+        "host_boundary": host,
         "roofline": roof, "kernels": kernels, "cpu_baseline": cpu,
     }
     emit(line)
@@ -310,6 +315,30 @@ def roofline(kname, prec, V, mean_ms, dims):
     if traffic:
         out["frac_counter"] = round(traffic / (mean_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
         out["traffic_per_site"] = round(traffic / V, 1)
+    return out
+
+
+def host_boundary(g, seed):
+    """GLU's entry points hand over host links: the first gauge copy once
+    more with the upload of its links from host memory and the download of
+    the result inside the clock (outside `value`, whose inputs are resident
+    in HBM)."""
+    g.restore()
+    g.random_transform(seed=seed)
+    U = g.get_links()
+    g.measure()
+    t0 = time.perf_counter()
+    g.set_links(U)
+    g.measure()  # synchronises the device
+    t1 = time.perf_counter()
+    it, th, nr = g.landau(0, 0.08, 1e-14, 1000, restart_seed=seed)
+    t2 = time.perf_counter()
+    U = g.get_links()
+    t3 = time.perf_counter()
+    out = {"one_copy_configs_per_s": round(1.0 / (t3 - t0), 4), "upload_ms": round(1e3 * (t1 - t0), 1),
+           "solve_ms": round(1e3 * (t2 - t1), 1), "download_ms": round(1e3 * (t3 - t2), 1),
+           "bytes_each_way": int(U.nbytes)}
+    print(f"[bench] with the host boundary: {out}", file=sys.stderr, flush=True)
     return out
 
 
