@@ -18,13 +18,13 @@
 // context path streams, and written back in GLU's layout at the end with
 // the third column completed.  The fields in, sn and in_old (4 packed
 // complex fields each, Gfix/FACG.c:57-60) live in the same scratch.
-// The Fourier acceleration runs the 4D transform as one direct DFT per axis
-// on pencils held in registers (extents up to 16), the p^2 factor between
-// the forward and backward passes.  Reductions are workgroup sums in a
-// fixed order.
+// The Fourier acceleration is a direct DFT per axis (extents up to 16): the
+// x, y, z sums of each time slice staged in LDS, the t sums with the p^2
+// factor between them on pencils in registers, so three passes over the
+// fields.  Reductions are workgroup sums in a fixed order.
 #include <algorithm>
-#include <cstdlib>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/glu_gfix.h"
@@ -152,11 +152,11 @@ template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict_
 
 // The x, y and z transforms of one time slice of one field (a slab of
 // Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
-// stage_cx<WG>() complex numbers hold: the slabs cross HBM once for the three axes
-// instead of three times.  Each axis is the same direct sum as bdft_axis,
-// in the same order, from one LDS buffer into the other.
-// 4096 (two 64 KiB buffers) with 512 threads, which hold a CU alone;
-// 2048 with fewer, whose workgroups share a CU
+// the stage holds: the slabs cross HBM once for the three axes instead of
+// three times.  Each axis is the same direct sum as bdft_axis, in the same
+// order, from one LDS buffer into the other.  The stage holds 4096 complex
+// numbers per buffer (two of 64 KiB) with 512 threads, whose workgroup
+// holds a CU alone, and 2048 with fewer, whose workgroups share a CU.
 template <int WG> constexpr int stage_cx() { return WG >= 512 ? 4096 : 2048; }
 
 template <int WG>
@@ -368,7 +368,7 @@ struct BTrace {
 // fields in, sn, old (4 V complex each).  Writes its iteration count (GLU's
 // return value) and final theta.
 template <int WG>
-__global__ __launch_bounds__(WG, 2) void k_facg_batch(double2 *const *__restrict__ lat, double2 *__restrict__ scratch,
+__global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *const *__restrict__ lat, double2 *__restrict__ scratch,
                                                           BGeo g, int nlat, double gf, double acc,
                                                           long long max_iters, long long *iters_out,
                                                           double *theta_out, double *trace, long long trace_cap,
