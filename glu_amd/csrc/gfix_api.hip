@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/glu_gfix.h"
@@ -267,6 +268,10 @@ extern "C" int glu_gfix_destroy(glu_gfix_ctx *c) {
     if (gx) (void)hipGraphExecDestroy(gx);
   if (c->run_ev) (void)hipEventDestroy(c->run_ev);
   if (c->hctrl) (void)hipHostFree(c->hctrl);
+  for (int b = 0; b < 2; b++) {
+    if (c->hstage[b]) (void)hipHostFree(c->hstage[b]);
+    if (c->hev[b]) (void)hipEventDestroy(c->hev[b]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return 0;
@@ -282,19 +287,117 @@ int ensure_stage(glu_gfix_ctx *c) {
   return 0;
 }
 
-extern "C" int glu_gfix_set_links(glu_gfix_ctx *c, const double *links) {
-  if (!c || !links) return gfix_fail(-1, "glu_gfix_set_links: null argument");
+// host threads for the gather / scatter of one chunk
+static int host_threads() {
+  static const int n = [] {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return (int)std::max(1u, std::min(hw ? hw : 1u, 16u));
+  }();
+  return n;
+}
+
+// f(s0, n) over [s0, s0 + n) split among the host threads
+template <typename F> static void par_sites(long long s0, long long n, F f) {
+  const int nt = (int)std::min<long long>(host_threads(), std::max<long long>(1, n / 2048));
+  if (nt <= 1) {
+    f(s0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const long long per = (n + nt - 1) / nt;
+  for (int t = 1; t < nt; t++) {
+    const long long a = s0 + t * per, m = std::min(per, s0 + n - a);
+    if (m > 0) th.emplace_back([=] { f(a, m); });
+  }
+  f(s0, std::min(per, n));
+  for (std::thread &x : th) x.join();
+}
+
+static int ensure_host_stage(glu_gfix_ctx *c) {
   if (int rc = ensure_stage(c)) return rc;
+  for (int b = 0; b < 2; b++) {
+    if (!c->hstage[b] && hipHostMalloc((void **)&c->hstage[b], (size_t)kHostChunk * 72 * sizeof(double)) != hipSuccess) {
+      c->hstage[b] = nullptr;
+      return gfix_fail(-2, "pinned staging allocation failed");
+    }
+    if (!c->hev[b] && hipEventCreateWithFlags(&c->hev[b], hipEventDisableTiming) != hipSuccess) {
+      c->hev[b] = nullptr;
+      return gfix_fail(-2, "staging event creation failed");
+    }
+  }
+  return 0;
+}
+
+int upload_links(glu_gfix_ctx *c, SiteGather gather, void *arg) {
+  if (int rc = ensure_host_stage(c)) return rc;
   const long long V = c->g.V;
-  for (long long s0 = 0; s0 < V; s0 += kChunk) {
-    const long long n = std::min(kChunk, V - s0);
-    if (hipMemcpyAsync(c->stage, links + s0 * 72, (size_t)n * 72 * sizeof(double), hipMemcpyHostToDevice,
-                       c->stream) != hipSuccess)
+  int k = 0;
+  for (long long s0 = 0; s0 < V; s0 += kHostChunk, k++) {
+    const long long n = std::min(kHostChunk, V - s0);
+    const int b = k & 1;
+    // the copy that read this half two chunks ago has finished
+    if (k >= 2 && hipEventSynchronize(c->hev[b]) != hipSuccess) return gfix_fail(-2, "link upload sync failed");
+    double *h = c->hstage[b];
+    par_sites(s0, n, [&](long long a, long long m) { gather(arg, h + (a - s0) * 72, a, m); });
+    if (hipMemcpyAsync(c->stage, h, (size_t)n * 72 * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+            hipSuccess ||
+        hipEventRecord(c->hev[b], c->stream) != hipSuccess)
       return gfix_fail(-2, "link upload failed");
     launch_pack<double>((const double *)c->stage, c->U64, V, 4, s0, n, c->stream);
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "link upload: %s", "sync failed");
   return 0;
+}
+
+int download_links(glu_gfix_ctx *c, SiteScatter scatter, void *arg) {
+  if (int rc = ensure_host_stage(c)) return rc;
+  const long long V = c->g.V;
+  // empty the half that chunk (s0, n) was copied into, once its copy is done
+  auto drain = [&](int b, long long s0, long long n) {
+    if (hipEventSynchronize(c->hev[b]) != hipSuccess) return gfix_fail(-2, "link download sync failed");
+    const double *h = c->hstage[b];
+    par_sites(s0, n, [&](long long a, long long m) { scatter(arg, h + (a - s0) * 72, a, m); });
+    return 0;
+  };
+  int k = 0;
+  long long prev0 = 0, prevn = 0;
+  for (long long s0 = 0; s0 < V; s0 += kHostChunk, k++) {
+    const long long n = std::min(kHostChunk, V - s0);
+    const int b = k & 1;
+    launch_unpack<double>(c->U64, (double *)c->stage, V, 4, s0, n, c->stream);
+    if (hipMemcpyAsync(c->hstage[b], c->stage, (size_t)n * 72 * sizeof(double), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipEventRecord(c->hev[b], c->stream) != hipSuccess)
+      return gfix_fail(-2, "link download failed");
+    // the previous chunk's half empties while this chunk is copied
+    if (k >= 1)
+      if (int rc = drain(b ^ 1, prev0, prevn)) return rc;
+    prev0 = s0;
+    prevn = n;
+  }
+  if (k >= 1)
+    if (int rc = drain((k - 1) & 1, prev0, prevn)) return rc;
+  return 0;
+}
+
+extern "C" int glu_gfix_set_links(glu_gfix_ctx *c, const double *links) {
+  if (!c || !links) return gfix_fail(-1, "glu_gfix_set_links: null argument");
+  return upload_links(
+      c,
+      [](void *arg, double *dst, long long s0, long long n) {
+        memcpy(dst, (const double *)arg + s0 * 72, (size_t)n * 72 * sizeof(double));
+      },
+      (void *)links);
+}
+
+extern "C" int glu_gfix_get_links(glu_gfix_ctx *c, double *links) {
+  if (!c || !links) return gfix_fail(-1, "glu_gfix_get_links: null argument");
+  return download_links(
+      c,
+      [](void *arg, const double *src, long long s0, long long n) {
+        memcpy((double *)arg + s0 * 72, src, (size_t)n * 72 * sizeof(double));
+      },
+      links);
 }
 
 extern "C" int glu_gfix_set_links_device(glu_gfix_ctx *c, const double *links_dev) {
@@ -314,21 +417,6 @@ extern "C" int glu_gfix_get_links_device(glu_gfix_ctx *c, double *links_dev) {
 extern "C" int glu_gfix_set_graph(glu_gfix_ctx *c, int on) {
   if (!c) return gfix_fail(-1, "null context");
   c->use_graph = on ? 1 : 0;
-  return 0;
-}
-
-extern "C" int glu_gfix_get_links(glu_gfix_ctx *c, double *links) {
-  if (!c || !links) return gfix_fail(-1, "glu_gfix_get_links: null argument");
-  if (int rc = ensure_stage(c)) return rc;
-  const long long V = c->g.V;
-  for (long long s0 = 0; s0 < V; s0 += kChunk) {
-    const long long n = std::min(kChunk, V - s0);
-    launch_unpack<double>(c->U64, (double *)c->stage, V, 4, s0, n, c->stream);
-    if (hipMemcpyAsync(links + s0 * 72, c->stage, (size_t)n * 72 * sizeof(double), hipMemcpyDeviceToHost,
-                       c->stream) != hipSuccess)
-      return gfix_fail(-2, "link download failed");
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return gfix_fail(-2, "link download sync failed");
-  }
   return 0;
 }
 

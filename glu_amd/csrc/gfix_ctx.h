@@ -107,6 +107,10 @@ struct glu_gfix_ctx {
   gfix::FftAxis ax[4]{};
   gfix::PsqInfo ps{};
   void *stage = nullptr;  // upload / download staging
+  // pinned host halves of the host transfers (kHostChunk sites each) and
+  // the events that free them (first host transfer)
+  double *hstage[2] = {nullptr, nullptr};
+  hipEvent_t hev[2] = {nullptr, nullptr};
   void *lie = nullptr;    // 16 packed Lie fields of the gluon propagators (first use)
   void *glue = nullptr;   // device copy of the last momentum list measured
   size_t last_fp32_iters = 0, last_fp64_iters = 0;
@@ -178,6 +182,17 @@ extern "C" int glu_gfix_reunit(glu_gfix_ctx *c);
 // staging buffer (kChunk * 72 doubles, allocated on first use)
 constexpr long long kChunk = 1 << 17;
 int ensure_stage(glu_gfix_ctx *c);
+// Host transfers of GLU-layout links, kHostChunk sites at a time through
+// two pinned buffers: host threads fill one (gather) or empty one
+// (scatter) while the device copies the other.  gather(dst, s0, n) writes
+// sites [s0, s0 + n) to dst (n x 72 doubles); scatter(src, s0, n) reads
+// them from src.  Both are called from several host threads at once, on
+// disjoint site ranges.
+constexpr long long kHostChunk = 1 << 16;
+using SiteGather = void (*)(void *arg, double *dst, long long s0, long long n);
+using SiteScatter = void (*)(void *arg, const double *src, long long s0, long long n);
+int upload_links(glu_gfix_ctx *c, SiteGather gather, void *arg);
+int download_links(glu_gfix_ctx *c, SiteScatter scatter, void *arg);
 int check_extents(const size_t dims[4], int gftype);
 void fft_plan(glu_gfix_ctx *c, double2 *const tw[4], double *cosv);
 void glueprop_free(glu_gfix_ctx *c);

@@ -55,16 +55,34 @@ glu_gfix_ctx *get_ctx(int gftype) {
   return g_ctx;
 }
 
-void pack(const struct site *lat, std::vector<double> &h) {
-  const size_t V = volume();
-  h.resize(V * 72);
-  for (size_t i = 0; i < V; i++)
-    for (int mu = 0; mu < 4; mu++) memcpy(&h[(i * 4 + mu) * 18], lat[i].O[mu], 18 * sizeof(double));
-}
 void unpack(struct site *lat, const std::vector<double> &h) {
   const size_t V = volume();
   for (size_t i = 0; i < V; i++)
     for (int mu = 0; mu < 4; mu++) memcpy(lat[i].O[mu], &h[(i * 4 + mu) * 18], 18 * sizeof(double));
+}
+
+// struct site links straight to and from the device, through the pinned,
+// double-buffered host transfers (several host threads gather or scatter
+// each chunk while the device copies the previous one)
+int sites_to_device(glu_gfix_ctx *c, const struct site *lat) {
+  return upload_links(
+      c,
+      [](void *arg, double *dst, long long s0, long long n) {
+        const struct site *l = (const struct site *)arg;
+        for (long long i = s0; i < s0 + n; i++)
+          for (int mu = 0; mu < 4; mu++) memcpy(dst + ((i - s0) * 4 + mu) * 18, l[i].O[mu], 18 * sizeof(double));
+      },
+      (void *)lat);
+}
+int device_to_sites(glu_gfix_ctx *c, struct site *lat) {
+  return download_links(
+      c,
+      [](void *arg, const double *src, long long s0, long long n) {
+        struct site *l = (struct site *)arg;
+        for (long long i = s0; i < s0 + n; i++)
+          for (int mu = 0; mu < 4; mu++) memcpy(l[i].O[mu], src + ((i - s0) * 4 + mu) * 18, 18 * sizeof(double));
+      },
+      lat);
 }
 
 // ------------------------------------------------------------------------
@@ -288,12 +306,12 @@ void log_herm(const cd *U, cd *Q) {
 
 // Gfix/gftests.c:26-130: average difference of the slice sums of the
 // traceless Hermitian projection of the temporal links from slice 0
-void const_time(const std::vector<double> &h, double *lin, double *lg) {
+void const_time(const struct site *lat, double *lin, double *lg) {
   const size_t Lcu = Latt.dims[0] * Latt.dims[1] * Latt.dims[2], T = Latt.dims[3];
   auto slice_sum = [&](size_t t, cd *s, cd *sl) {
     for (int k = 0; k < 9; k++) s[k] = sl[k] = 0.0;
     for (size_t j = Lcu * t; j < Lcu * t + Lcu; j++) {
-      const cd *U = (const cd *)&h[(j * 4 + 3) * 18];
+      const cd *U = (const cd *)lat[j].O[3];
       cd A[9];
       const double i0 = U[0].imag(), i4 = U[4].imag(), i8 = U[8].imag();
       A[0] = (2 * i0 - i4 - i8) / 3.0;
@@ -328,13 +346,11 @@ void output_fixing_info(glu_gfix_ctx *c, struct site *lat, double theta, size_t 
   glu_gfix_reunit(c);
   double plaq = 0, tr = 0, mx = 0, func = 0, tl = 0;
   glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
-  std::vector<double> h(volume() * 72);
-  glu_gfix_get_links(c, h.data());
-  unpack(lat, h);
+  if (device_to_sites(c, lat)) fprintf(stderr, "[GF] link download failed: %s\n", glu_gfix_last_error());
   printf("[GF] Plaquette :: %1.15f \n[GF] Accuracy :: %1.4e\n", plaq, theta);
   printf("[GF] Iters :: %zu\n[GF] Link trace :: %1.15f || Maximum :: %1.15f\n", iters, tr, mx);
   double lin = 0, lg = 0;
-  const_time(h, &lin, &lg);
+  const_time(lat, &lin, &lg);
   printf("[GF] Temporal constance || Lin %e || Log %e \n", lin, lg);
   printf("[GF] Functional :: %1.15f\n", func);
 }
@@ -348,12 +364,13 @@ int luxury_copies() {
   return n > 0 ? n : 20;
 }
 
-// FA_callback of Gfix/Landau.c:225-237: FACG, FASD, or luxury_copy_fast
-size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int method, bool callback) {
+// FA_callback of Gfix/Landau.c:225-237: FACG, FASD, or luxury_copy_fast.
+// The fixed links stay on the device, and in lat too unless the caller
+// takes them later (download false: Landau() reunitarises them first)
+size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int method, bool callback,
+               bool download = true) {
   glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
   if (!c) return 123456789;
-  std::vector<double> h;
-  pack(lat, h);
   size_t iters = 0;
   const int lux = callback ? luxury_copies() : 0;
   // the copies draw from GLU's process stream, continued on the device
@@ -361,15 +378,17 @@ size_t fa_call(struct site *lat, double *th, double acc, size_t max_iters, int m
     fprintf(stderr, "[GF] random stream unavailable: %s\n", glu_gfix_last_error());
     return 123456789;
   }
-  if (glu_gfix_set_links(c, h.data()) ||
+  if (sites_to_device(c, lat) ||
       (lux ? glu_gfix_luxury(c, method, Latt.gf_alpha, acc, max_iters, lux, 0, th, &iters, nullptr) ||
                  mwc_to_host(c)
            : glu_gfix_run(c, method, Latt.gf_alpha, acc, max_iters, th, &iters))) {
     fprintf(stderr, "[GF] device gauge fixing failed: %s\n", glu_gfix_last_error());
     return 123456789;
   }
-  glu_gfix_get_links(c, h.data());
-  unpack(lat, h);
+  if (download && device_to_sites(c, lat)) {
+    fprintf(stderr, "[GF] link download failed: %s\n", glu_gfix_last_error());
+    return 123456789;
+  }
   return iters;
 }
 
@@ -389,7 +408,7 @@ extern "C" size_t FASD(struct site *lat, struct fftw_stuff *FFTW, double *th, co
 extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t max_iter, const char *infile) {
   const int method = getenv("GLU_GFIX_SD") ? GLU_GFIX_SD : GLU_GFIX_CG;
   double theta = 0.0;
-  size_t iters = fa_call(lat, &theta, accuracy, max_iter, method, true);
+  size_t iters = fa_call(lat, &theta, accuracy, max_iter, method, true, false);
   size_t failure = 0;
   if (iters == 123456789) {
     fprintf(stderr, "\n[GF] Non-convergence ... Randomly-restarting \n\n[GF] Failure :: %zu || Iters :: %zu || ACC :: %e \n",
@@ -405,7 +424,7 @@ extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t m
         break;
       }
       unpack(lat, h);
-      const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method, true);
+      const size_t loc = fa_call(lat, &theta, accuracy, max_iter, method, true, false);
       if (loc != 123456789) {
         iters += loc;
         break;
@@ -415,18 +434,16 @@ extern "C" size_t Landau(struct site *lat, const double accuracy, const size_t m
               failure, loc, theta);
     }
   }
+  glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
   if (failure == 7) {
     printf("\n[GF] Failure to converge to a sufficient solution \n[GF] Please lower the GF_TUNE parameter from %g \n",
            Latt.gf_alpha);
+    if (c) (void)device_to_sites(c, lat);  // lat holds the last attempt, as GLU leaves it
     return (size_t)-1;
   }
-  glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
-  if (c) {
-    std::vector<double> h;
-    pack(lat, h);
-    glu_gfix_set_links(c, h.data());
-    output_fixing_info(c, lat, theta, iters);
-  }
+  // the context holds the fixed links: reunitarised, reported and
+  // written back into lat
+  if (c) output_fixing_info(c, lat, theta, iters);
   return iters;
 }
 
@@ -435,13 +452,11 @@ extern "C" size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW, const do
                              const GLU_bool facg) {
   glu_gfix_ctx *c = get_ctx(GLU_GFIX_COULOMB);
   if (!c) return 0;
-  std::vector<double> h;
-  pack(lat, h);
   size_t iters = 0;
   double th = 0.0;
   // initialise_par_rng on entry, free_par_rng on return (Gfix/CFACG.c:539,
   // 630): the slice restarts continue GLU's process stream on the device
-  const int rc = rng_init() || mwc_from_host(c) || glu_gfix_set_links(c, h.data()) ||
+  const int rc = rng_init() || mwc_from_host(c) || sites_to_device(c, lat) ||
                  glu_gfix_run(c, facg == GLU_TRUE ? GLU_GFIX_CG : GLU_GFIX_SD, Latt.gf_alpha, accuracy, max_iter,
                               &th, &iters);
   mwc_host_free();
@@ -449,8 +464,7 @@ extern "C" size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW, const do
     fprintf(stderr, "[GF] device Coulomb gauge fixing failed: %s\n", glu_gfix_last_error());
     return 0;
   }
-  glu_gfix_get_links(c, h.data());
-  unpack(lat, h);
+  if (device_to_sites(c, lat)) fprintf(stderr, "[GF] link download failed: %s\n", glu_gfix_last_error());
   return iters;
 }
 
@@ -458,20 +472,16 @@ extern "C" size_t Coulomb_FA(struct site *lat, struct fftw_stuff *FFTW, const do
 extern "C" size_t Coulomb(struct site *lat, const double accuracy, const size_t iter) {
   glu_gfix_ctx *c = get_ctx(GLU_GFIX_COULOMB);
   if (!c) return 0;
-  std::vector<double> h;
-  pack(lat, h);
-  glu_gfix_set_links(c, h.data());
+  if (sites_to_device(c, lat)) fprintf(stderr, "[GF] link upload failed: %s\n", glu_gfix_last_error());
   double plaq = 0, tr = 0, mx = 0, func = 0, tl = 0;
   glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
   double sl = (4 * tr - tl) / 3.0;
   fprintf(stdout, "[GF] Initial Tlink :: %1.15f || Slink :: %1.15f \n[GF] Plaquette :: %1.15f \n", tl, sl, plaq);
   const size_t iters =
       Coulomb_FA(lat, nullptr, accuracy, iter, getenv("GLU_GFIX_SD") ? GLU_FALSE : GLU_TRUE);
-  pack(lat, h);
-  glu_gfix_set_links(c, h.data());
+  // the context still holds the fixed links Coulomb_FA wrote into lat
   glu_gfix_reunit(c);
-  glu_gfix_get_links(c, h.data());
-  unpack(lat, h);
+  if (device_to_sites(c, lat)) fprintf(stderr, "[GF] link download failed: %s\n", glu_gfix_last_error());
   glu_gfix_measure(c, &plaq, &tr, &mx, &func, &tl);
   sl = (4 * tr - tl) / 3.0;
   fprintf(stdout,
@@ -526,13 +536,11 @@ extern "C" int cuts_struct(struct site *A, const struct cut_info CUTINFO) {
   }
   glu_gfix_ctx *c = get_ctx(GLU_GFIX_LANDAU);
   if (!c) return kFailure;
-  std::vector<double> h;
-  pack(A, h);
   const std::string path = cut_file(CUTINFO);
   fprintf(stdout, "[CUTS] Our file is located @ %s \n", CUTINFO.where);
   size_t nmom = 0;
   const double angle = (double)CUTINFO.angle;
-  int rc = glu_gfix_set_links(c, h.data());
+  int rc = sites_to_device(c, A);
   if (!rc)
     rc = CUTINFO.dir == GLUON_PROPS
              ? glu_gfix_gluon_props(c, (int)CUTINFO.type, CUTINFO.max_mom, CUTINFO.cyl_width, angle, path.c_str(),
