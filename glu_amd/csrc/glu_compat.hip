@@ -6,12 +6,14 @@
 // Gfix/Coulomb.c:33-100, Gfix/FACG.c:341-446): the same theta test,
 // continuation, random restarts and [GF] report lines.  The work itself
 // runs on the device through the glu_gfix_* context API.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/glu_gfix.h"
@@ -325,12 +327,23 @@ void const_time(const struct site *lat, double *lin, double *lg) {
       for (int k = 0; k < 9; k++) sl[k] += Q[k];
     }
   };
-  cd s1[9], l1[9];
-  slice_sum(0, s1, l1);
+  // each slice's sums in its own order, the slices on host threads (an
+  // exact logarithm per temporal link is the cost); then combined in slice
+  // order, so the result does not depend on the thread count
+  std::vector<cd> S(T * 9), SL(T * 9);
+  {
+    const size_t nt = std::min<size_t>(T, std::max(1u, std::min(std::thread::hardware_concurrency(), 16u)));
+    std::vector<std::thread> th;
+    for (size_t w = 0; w < nt; w++)
+      th.emplace_back([&, w] {
+        for (size_t t = w; t < T; t += nt) slice_sum(t, &S[t * 9], &SL[t * 9]);
+      });
+    for (std::thread &x : th) x.join();
+  }
+  const cd *s1 = &S[0], *l1 = &SL[0];
   double av = 0, avl = 0;
   for (size_t t = 1; t < T; t++) {
-    cd s2[9], l2[9];
-    slice_sum(t, s2, l2);
+    const cd *s2 = &S[t * 9], *l2 = &SL[t * 9];
     av += fabs(s1[0].real() - s2[0].real()) + std::abs(s1[1] - s2[1]) + std::abs(s1[2] - s2[2]) +
           fabs(s1[4].real() - s2[4].real()) + std::abs(s1[5] - s2[5]);
     avl += fabs(l1[0].real() - l2[0].real()) + std::abs(l1[1] - l2[1]) + std::abs(l1[2] - l2[2]) +
