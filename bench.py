@@ -14,8 +14,9 @@ line of their own: --workload coulomb (48^3x96 Coulomb FACG, the 96 time
 slices batched), --workload serve (a Poisson stream of 64 Landau
 requests of mixed sizes through the serving loop), --workload glueprop
 (gluon propagators of a Landau-fixed 32^3x64 config, GLU's GLUON_PROPS
-cuts) and --workload batch (512 gauge copies of an 8^4 config fixed in one
-launch, one workgroup per lattice).
+cuts), --workload batch (512 gauge copies of an 8^4 config fixed in one
+launch, one workgroup per lattice) and --workload dropin (GLU's own
+Landau() on host struct site links, FP64 and mixed).
 """
 from __future__ import annotations
 
@@ -70,7 +71,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=20,
                     help="CPU baseline sample: k and 2k oracle iterations")
-    ap.add_argument("--workload", choices=("landau", "coulomb", "serve", "glueprop", "batch"), default="landau")
+    ap.add_argument("--workload", choices=("landau", "coulomb", "serve", "glueprop", "batch", "dropin"),
+                    default="landau")
     ap.add_argument("--batch", type=int, default=512, help="batch: lattices per launch")
     ap.add_argument("--max-mom", type=int, default=64, help="glueprop: PSQ cut p^2 <= max_mom")
     ap.add_argument("--graph", action="store_true", help="replay iteration batches from HIP graphs")
@@ -180,6 +182,8 @@ def main():
         return glueprop_main(a)
     if a.workload == "batch":
         return batch_main(a)
+    if a.workload == "dropin":
+        return dropin_main(a)
     dims = tuple(int(x) for x in a.dims.split(","))
     V = math.prod(dims)
     print(f"[bench] lattice {dims} V={V}: generating input "
@@ -380,6 +384,78 @@ COULOMB_KERNEL_BYTES = {
     "c_line_search": (576 + 144 + 64, 288 + 96 + 64), "c_slice_update": (144 + 64 + 144, 96 + 64 + 96),
     "fft_axis": (128, 128),
 }
+
+
+def dropin_main(a):
+    """GLU's own Landau() (Gfix/Landau.c:261-374 prototype): `struct site`
+    links in host memory and the global Latt, as GLU calls it.  One call =
+    the links gathered and uploaded, fixed (FP64, then mixed through
+    GLU_GFIX_PRECISION), reunitarised, reported with GLU's [GF] lines
+    (including its temporal-constancy logarithms) and written back into the
+    host links.  The first call of each precision (context creation) is not
+    timed."""
+    import ctypes as C
+
+    import glu_amd as gl
+
+    dims = tuple(int(x) for x in a.dims.split(","))
+    V = math.prod(dims)
+    n = max(1, min(a.steps, 8))
+    g = gl.Gfix(dims, gl.LANDAU, gl.FP64)
+    g.heatbath(6.0, a.sweeps, 4, seed=a.seed, cold_start=True)
+    plaq = g.measure()["plaquette"]
+    g.snapshot()
+    lib = C.CDLL(gl.LIB_PATH)
+    lib.Landau.restype = C.c_size_t
+    lib.Landau.argtypes = [C.c_void_p, C.c_double, C.c_size_t, C.c_char_p]
+
+    class LattInfo(C.Structure):  # struct latt_info, Headers/GLU_types.h:229-243
+        _fields_ = [("dims", C.c_size_t * 4), ("Lsq", C.c_size_t), ("Lcu", C.c_size_t), ("Volume", C.c_size_t),
+                    ("flow", C.c_size_t), ("gf_alpha", C.c_double), ("sm_alpha", C.c_double * 3),
+                    ("head", C.c_int), ("Seed", C.c_uint32 * 1), ("twiddles", C.c_double * 4),
+                    ("Nthreads", C.c_uint32), ("su2_data", C.c_void_p), ("argc", C.c_int)]
+
+    L = LattInfo.in_dll(lib, "Latt")
+    for mu in range(4):
+        L.dims[mu] = dims[mu]
+    L.Lsq, L.Lcu, L.Volume = dims[0] * dims[1], dims[0] * dims[1] * dims[2], V
+    L.gf_alpha, L.Seed[0], L.Nthreads = 0.08, 1, 1
+    U = np.empty((V, 4, 18))
+    # struct site: O -> 4 pointers, one per direction, to 9 complex numbers
+    ptr = (U.ctypes.data + 144 * np.arange(V * 4, dtype=np.uint64)).reshape(V, 4)
+    site = np.zeros(V, np.dtype([("O", "u8"), ("neighbor", "u8", (4,)), ("back", "u8", (4,))]))
+    site["O"] = ptr.ctypes.data + 32 * np.arange(V, dtype=np.uint64)
+    runs = {}
+    for prec in ("fp64", "mixed"):
+        os.environ["GLU_GFIX_PRECISION"] = prec
+        times, its = [], []
+        for k in range(n + 1):
+            g.restore()
+            g.random_transform(seed=a.seed + 1 + k)
+            U[...] = g.get_links()
+            t = time.perf_counter()
+            it = lib.Landau(site.ctypes.data, 1e-14, 1000, b"")
+            dt = time.perf_counter() - t
+            if k:
+                times.append(dt)
+                its.append(int(it))
+        runs[prec] = {"calls_per_s": round(len(times) / sum(times), 4), "s_per_call": round(sum(times) / len(times), 3),
+                      "iters": its}
+        print(f"[bench] Landau() {prec}: {runs[prec]}", file=sys.stderr, flush=True)
+    os.environ.pop("GLU_GFIX_PRECISION", None)
+    line = {
+        "metric": f"GLU Landau() calls/s at {'x'.join(map(str, dims))} (host struct site links in and out, report "
+                  "included)",
+        "value": runs["mixed"]["calls_per_s"], "unit": "calls/s", "n_gpus": 1, "steps": n, "warmup": 1,
+        "ms_per_step": round(1e3 * runs["mixed"]["s_per_call"], 1), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+fp64",
+        "data": (f"synthetic: device quenched heatbath beta=6.0 ({a.sweeps} sweeps from cold, seed {a.seed}); call k "
+                 f"fixes gauge copy seed {a.seed + 1}+k, copied into GLU-layout host links"),
+        "config": {"workload": "glu_landau_dropin_" + "x".join(map(str, dims)), "lattice": list(dims),
+                   "accuracy": 1e-14, "max_iters": 1000, "gf_tune": 0.08},
+        "plaquette": plaq, "runs": runs, "roofline": None, "cpu_baseline": None,
+    }
+    emit(line)
 
 
 def batch_main(a):

@@ -51,6 +51,12 @@ glu_gfix_ctx *get_ctx(int gftype) {
     return nullptr;
   }
   g_ctx->verbose = 1;  // GLU's [GF] / [CG] progress lines
+  // launch shapes timed once per shape and process (GLU_GFIX_AUTOTUNE=1):
+  // worth it when a process fixes several configurations, as a GLU run
+  // over many files does; the defaults serve a single call
+  if (const char *e = getenv("GLU_GFIX_AUTOTUNE"))
+    if (atoi(e) > 0 && glu_gfix_autotune(g_ctx, nullptr, 0) != 0)
+      fprintf(stderr, "[GF] autotune failed (defaults kept): %s\n", glu_gfix_last_error());
   memcpy(g_dims, Latt.dims, sizeof(g_dims));
   g_type = gftype;
   g_prec = prec;
@@ -267,13 +273,21 @@ void log_herm(const cd *U, cd *Q) {
   const cd d = U[0] * (U[4] * U[8] - U[5] * U[7]) - U[1] * (U[3] * U[8] - U[5] * U[6]) +
                U[2] * (U[3] * U[7] - U[4] * U[6]);
   cd l[3] = {cd(1, 0), cd(-0.5, 0.8660254037844386), cd(-0.5, -0.8660254037844386)};
-  for (int it = 0; it < 200; it++) {  // Durand-Kerner
+  // Durand-Kerner, until every correction is below 1e-15 (the eigenvalues
+  // lie on the unit circle; convergence is quadratic, a dozen sweeps)
+  for (int it = 0; it < 200; it++) {
+    double step = 0.0;
     for (int k = 0; k < 3; k++) {
       cd num = ((l[k] - a) * l[k] + b) * l[k] - d, den = 1.0;
       for (int j = 0; j < 3; j++)
         if (j != k) den *= (l[k] - l[j]);
-      if (std::abs(den) > 0) l[k] -= num / den;
+      if (std::abs(den) > 0) {
+        const cd c = num / den;
+        l[k] -= c;
+        step = std::max(step, std::abs(c));
+      }
     }
+    if (step < 1e-15) break;
   }
   for (int k = 0; k < 9; k++) Q[k] = 0.0;
   for (int k = 0; k < 3; k++) {
