@@ -264,47 +264,69 @@ int read_nersc(const char *path, std::vector<double> &h) {
 // ------------------------------------------------------------------------
 // report helpers
 // ------------------------------------------------------------------------
+// complex arithmetic of the report's logarithm, written out: the library
+// multiply and divide of std::complex take the C99 Annex G slow path for
+// non-finite values on every call, and 2 x 10^6 logarithms per report add
+// up; the values are finite here and the formulas the usual ones
+struct rc {
+  double re, im;
+};
+inline rc operator+(rc a, rc b) { return {a.re + b.re, a.im + b.im}; }
+inline rc operator-(rc a, rc b) { return {a.re - b.re, a.im - b.im}; }
+inline rc operator*(rc a, rc b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+inline rc operator*(double s, rc a) { return {s * a.re, s * a.im}; }
+inline rc conjr(rc a) { return {a.re, -a.im}; }
+inline double normr(rc a) { return a.re * a.re + a.im * a.im; }
+inline rc operator/(rc a, rc b) {
+  const double n = normr(b);
+  return {(a.re * b.re + a.im * b.im) / n, (a.im * b.re - a.re * b.im) / n};
+}
+
 // -i log(U) of a unitary 3x3 matrix from its eigen-decomposition (the
 // "Log" column of the temporal-constancy line; GLU uses exact_log_slow)
-void log_herm(const cd *U, cd *Q) {
+void log_herm(const cd *Uc, cd *Qc) {
+  rc U[9];
+  for (int e = 0; e < 9; e++) U[e] = {Uc[e].real(), Uc[e].imag()};
   // characteristic polynomial l^3 - a l^2 + b l - d
-  const cd a = U[0] + U[4] + U[8];
-  const cd b = U[0] * U[4] - U[1] * U[3] + U[0] * U[8] - U[2] * U[6] + U[4] * U[8] - U[5] * U[7];
-  const cd d = U[0] * (U[4] * U[8] - U[5] * U[7]) - U[1] * (U[3] * U[8] - U[5] * U[6]) +
+  const rc a = U[0] + U[4] + U[8];
+  const rc b = U[0] * U[4] - U[1] * U[3] + U[0] * U[8] - U[2] * U[6] + U[4] * U[8] - U[5] * U[7];
+  const rc d = U[0] * (U[4] * U[8] - U[5] * U[7]) - U[1] * (U[3] * U[8] - U[5] * U[6]) +
                U[2] * (U[3] * U[7] - U[4] * U[6]);
-  cd l[3] = {cd(1, 0), cd(-0.5, 0.8660254037844386), cd(-0.5, -0.8660254037844386)};
+  rc l[3] = {{1, 0}, {-0.5, 0.8660254037844386}, {-0.5, -0.8660254037844386}};
   // Durand-Kerner, until every correction is below 1e-15 (the eigenvalues
   // lie on the unit circle; convergence is quadratic, a dozen sweeps)
   for (int it = 0; it < 200; it++) {
     double step = 0.0;
     for (int k = 0; k < 3; k++) {
-      cd num = ((l[k] - a) * l[k] + b) * l[k] - d, den = 1.0;
+      const rc num = ((l[k] - a) * l[k] + b) * l[k] - d;
+      rc den = {1, 0};
       for (int j = 0; j < 3; j++)
-        if (j != k) den *= (l[k] - l[j]);
-      if (std::abs(den) > 0) {
-        const cd c = num / den;
-        l[k] -= c;
-        step = std::max(step, std::abs(c));
+        if (j != k) den = den * (l[k] - l[j]);
+      if (normr(den) > 0) {
+        const rc c = num / den;
+        l[k] = l[k] - c;
+        step = std::max(step, std::sqrt(normr(c)));
       }
     }
     if (step < 1e-15) break;
   }
-  for (int k = 0; k < 9; k++) Q[k] = 0.0;
+  rc Q[9];
+  for (int k = 0; k < 9; k++) Q[k] = {0, 0};
   for (int k = 0; k < 3; k++) {
     // eigenvector: cross product of two rows of (U - l I)
-    cd M[9];
+    rc M[9];
     for (int e = 0; e < 9; e++) M[e] = U[e];
-    M[0] -= l[k];
-    M[4] -= l[k];
-    M[8] -= l[k];
-    cd best[3] = {0, 0, 0};
+    M[0] = M[0] - l[k];
+    M[4] = M[4] - l[k];
+    M[8] = M[8] - l[k];
+    rc best[3] = {{0, 0}, {0, 0}, {0, 0}};
     double bn = -1;
     for (int r1 = 0; r1 < 3; r1++)
       for (int r2 = r1 + 1; r2 < 3; r2++) {
-        const cd *p = M + 3 * r1, *q = M + 3 * r2;
-        cd v[3] = {std::conj(p[1] * q[2] - p[2] * q[1]), std::conj(p[2] * q[0] - p[0] * q[2]),
-                   std::conj(p[0] * q[1] - p[1] * q[0])};
-        const double nn = std::norm(v[0]) + std::norm(v[1]) + std::norm(v[2]);
+        const rc *p = M + 3 * r1, *q = M + 3 * r2;
+        const rc v[3] = {conjr(p[1] * q[2] - p[2] * q[1]), conjr(p[2] * q[0] - p[0] * q[2]),
+                         conjr(p[0] * q[1] - p[1] * q[0])};
+        const double nn = normr(v[0]) + normr(v[1]) + normr(v[2]);
         if (nn > bn) {
           bn = nn;
           best[0] = v[0];
@@ -314,10 +336,11 @@ void log_herm(const cd *U, cd *Q) {
       }
     if (bn <= 0) continue;
     const double s = 1.0 / sqrt(bn);
-    const double th = std::arg(l[k]);
+    const double th = std::atan2(l[k].im, l[k].re);
     for (int r = 0; r < 3; r++)
-      for (int c2 = 0; c2 < 3; c2++) Q[r * 3 + c2] += th * (best[r] * s) * std::conj(best[c2] * s);
+      for (int c2 = 0; c2 < 3; c2++) Q[r * 3 + c2] = Q[r * 3 + c2] + th * ((s * best[r]) * conjr(s * best[c2]));
   }
+  for (int e = 0; e < 9; e++) Qc[e] = cd(Q[e].re, Q[e].im);
 }
 
 // Gfix/gftests.c:26-130: average difference of the slice sums of the
