@@ -43,17 +43,26 @@ constexpr double kPl1 = 0.15, kPl2 = 0.30;  // Gfix/line_search.c:34-35
 constexpr long long kScratchPerSite = 36;
 constexpr int kMaxSegments = 100;  // continuation segments; GLU does not bound them
 
+// mL, mS: multipliers for division by the extents and strides without a
+// divide instruction (no GPU has one for integers): for 2 <= d and
+// x < 2^32 / d, x / d == umulhi(x, floor((2^32 - 1) / d) + 1).  Lattices
+// of at most 16^4 sites keep every quotient here in that range.
 struct BGeo {
   int L[4];
   long long S[4];
   long long V;
+  unsigned mL[4], mS[4];  // mS[0] unused (S[0] = 1)
 };
 
+__device__ __forceinline__ unsigned bquot(unsigned x, unsigned m) { return __umulhi(x, m); }
+
 __device__ __forceinline__ void bcoords(const BGeo &g, long long i, int x[4]) {
-  long long r = i;
+  unsigned r = (unsigned)i;
+#pragma unroll
   for (int mu = 0; mu < 4; mu++) {
-    x[mu] = (int)(r % g.L[mu]);
-    r /= g.L[mu];
+    const unsigned q = bquot(r, g.mL[mu]);
+    x[mu] = (int)(r - q * (unsigned)g.L[mu]);
+    r = q;
   }
 }
 __device__ __forceinline__ long long bfwd(const BGeo &g, long long i, const int x[4], int mu) {
@@ -126,22 +135,25 @@ template <int WG> __device__ __forceinline__ double bderiv(const double2 *__rest
 // forward, +1 backward, unnormalised); tw[k] = exp(-2 pi i k / N)
 template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict__ F, const BGeo &g, int mu, int sign, const double2 *tw) {
   const int N = g.L[mu];
-  const long long S = g.S[mu], V = g.V;
-  const long long np = V / N;  // pencils per field
-  for (long long p = threadIdx.x; p < 4 * np; p += WG) {
-    const long long f = p / np, q = p % np;
-    const long long base = f * V + (q % S) + (q / S) * S * N;
+  const unsigned S = (unsigned)g.S[mu], V = (unsigned)g.V;
+  const unsigned np = V / N;  // pencils per field
+  for (unsigned p = threadIdx.x; p < 4 * np; p += WG) {
+    const unsigned f = p / np, q = p % np;
+    const long long base = (long long)f * V + (q % S) + (q / S) * S * N;
     cx<double> a[kMaxExtent];
 #pragma unroll
     for (int j = 0; j < kMaxExtent; j++)
       if (j < N) a[j] = ld<double>(F + base + j * S);
     for (int k = 0; k < N; k++) {
       cx<double> acc = mk<double>(0.0, 0.0);
+      int jk = 0;  // j k mod N
 #pragma unroll
       for (int j = 0; j < kMaxExtent; j++) {
         if (j < N) {
-          const double2 w = tw[(j * k) % N];
+          const double2 w = tw[jk];
           acc = cadd(acc, cmul(a[j], mk<double>(w.x, sign < 0 ? w.y : -w.y)));
+          jk += k;
+          if (jk >= N) jk -= N;
         }
       }
       st<double>(F + base + k * S, acc);
@@ -161,20 +173,32 @@ template <int WG> constexpr int stage_cx() { return WG >= 512 ? 4096 : 2048; }
 
 template <int WG>
 __device__ __forceinline__ void lds_dft_axis(const double2 *__restrict__ in, double2 *__restrict__ out, int npts,
-                                             int s3, int N, int S, int sign, const double2 *tw) {
-  for (int p = threadIdx.x; p < npts; p += WG) {
-    const int k = ((p % s3) / S) % N;  // this point's index along the axis
-    const int base = p - k * S;        // the pencil's first point
-    cx<double> acc = mk<double>(0.0, 0.0);
+                                             int N, unsigned mN, int S, unsigned mS, int sign, const double2 *tw) {
+  // two points per thread, p and p + WG, whose sums are independent
+  // chains the SIMD interleaves (the last may repeat p, unstored)
+  for (int p = threadIdx.x; p < npts; p += 2 * WG) {
+    const int q = p + WG < npts ? p + WG : p;
+    // a point's index along the axis: (p / S) mod N, a slab's size being
+    // a multiple of S N
+    const unsigned ps = S == 1 ? (unsigned)p : bquot((unsigned)p, mS);
+    const unsigned qs = S == 1 ? (unsigned)q : bquot((unsigned)q, mS);
+    const int kp = (int)(ps - bquot(ps, mN) * (unsigned)N), kq = (int)(qs - bquot(qs, mN) * (unsigned)N);
+    const int bp = p - kp * S, bq = q - kq * S;  // the pencils' first points
+    cx<double> ap = mk<double>(0.0, 0.0), aq = mk<double>(0.0, 0.0);
+    int jp = 0, jq = 0;  // j k mod N
 #pragma unroll 4
     for (int j = 0; j < N; j++) {
-      {
-        const double2 w = tw[(j * k) % N];
-        const double2 a = in[base + j * S];
-        acc = cadd(acc, cmul(mk<double>(a.x, a.y), mk<double>(w.x, sign < 0 ? w.y : -w.y)));
-      }
+      const double2 wp = tw[jp], wq = tw[jq];
+      const double2 xp = in[bp + j * S], xq = in[bq + j * S];
+      ap = cadd(ap, cmul(mk<double>(xp.x, xp.y), mk<double>(wp.x, sign < 0 ? wp.y : -wp.y)));
+      aq = cadd(aq, cmul(mk<double>(xq.x, xq.y), mk<double>(wq.x, sign < 0 ? wq.y : -wq.y)));
+      jp += kp;
+      if (jp >= N) jp -= N;
+      jq += kq;
+      if (jq >= N) jq -= N;
     }
-    out[p] = make_double2(acc.re, acc.im);
+    out[p] = make_double2(ap.re, ap.im);
+    if (q != p) out[q] = make_double2(aq.re, aq.im);
   }
   __syncthreads();
 }
@@ -192,9 +216,9 @@ __device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g,
     double2 *src = F + (long long)s0 * s3;  // slab (f, t) starts at (f T + t) s3
     for (int p = threadIdx.x; p < npts; p += WG) b0[p] = src[p];
     __syncthreads();
-    lds_dft_axis<WG>(b0, b1, npts, s3, g.L[0], 1, sign, tw[0]);
-    lds_dft_axis<WG>(b1, b0, npts, s3, g.L[1], (int)g.S[1], sign, tw[1]);
-    lds_dft_axis<WG>(b0, b1, npts, s3, g.L[2], (int)g.S[2], sign, tw[2]);
+    lds_dft_axis<WG>(b0, b1, npts, g.L[0], g.mL[0], 1, 0u, sign, tw[0]);
+    lds_dft_axis<WG>(b1, b0, npts, g.L[1], g.mL[1], (int)g.S[1], g.mS[1], sign, tw[1]);
+    lds_dft_axis<WG>(b0, b1, npts, g.L[2], g.mL[2], (int)g.S[2], g.mS[2], sign, tw[2]);
     for (int p = threadIdx.x; p < npts; p += WG) src[p] = b1[p];
     // the next round's loads write b0, which no thread reads after the
     // barrier that ended the z sums
@@ -206,15 +230,16 @@ __device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g,
 // sums -- the middle of the Fourier acceleration in one pass.  The scaled
 // forward values go back to the pencil's own sites and are read again for
 // the backward sums (the thread's own stores, still in cache), so that
-// every register array is indexed by constants.
+// every register array is indexed by constants.  Outputs k and k + 1 are
+// summed together, two independent chains over the same inputs.
 template <int WG>
 __device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo &g, const double2 (*tw)[kMaxExtent]) {
   const int N = g.L[3];
   const long long S = g.S[3], V = g.V;
   const double mx = 4.0 * 4 / (double)V;
-  for (long long p = threadIdx.x; p < 4 * S; p += WG) {
-    const long long f = p / S, q = p % S;
-    const long long base = f * V + q;
+  for (unsigned p = threadIdx.x; p < 4 * S; p += WG) {
+    const unsigned f = bquot(p, g.mS[3]), q = p - f * (unsigned)S;
+    const long long base = (long long)f * V + q;
     int x[4];
     bcoords(g, q, x);
     const double kxyz = tw[0][x[0]].x + tw[1][x[1]].x + tw[2][x[2]].x;
@@ -222,32 +247,52 @@ __device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo
 #pragma unroll
     for (int j = 0; j < kMaxExtent; j++)
       if (j < N) a[j] = ld<double>(F + base + j * S);
-    for (int k = 0; k < N; k++) {
-      cx<double> acc = mk<double>(0.0, 0.0);
+    for (int k = 0; k < N; k += 2) {
+      const int k2 = k + 1 < N ? k + 1 : k;
+      cx<double> acc = mk<double>(0.0, 0.0), acc2 = acc;
+      int jk = 0, jk2 = 0;  // j k mod N
 #pragma unroll
       for (int j = 0; j < kMaxExtent; j++) {
         if (j < N) {
-          const double2 w = tw[3][(j * k) % N];
+          const double2 w = tw[3][jk], w2 = tw[3][jk2];
           acc = cadd(acc, cmul(a[j], mk<double>(w.x, w.y)));
+          acc2 = cadd(acc2, cmul(a[j], mk<double>(w2.x, w2.y)));
+          jk += k;
+          if (jk >= N) jk -= N;
+          jk2 += k2;
+          if (jk2 >= N) jk2 -= N;
         }
       }
       const double kc = kxyz + tw[3][k].x;  // sum of cos(2 pi n_mu / L_mu), mu in order
       const double fac = mx / ((kc == 4.0) ? 1.0 : 2.0 * (4.0 - kc));
       st<double>(F + base + k * S, mk<double>(acc.re * fac, acc.im * fac));
+      if (k2 != k) {
+        const double kc2 = kxyz + tw[3][k2].x;
+        const double fac2 = mx / ((kc2 == 4.0) ? 1.0 : 2.0 * (4.0 - kc2));
+        st<double>(F + base + k2 * S, mk<double>(acc2.re * fac2, acc2.im * fac2));
+      }
     }
 #pragma unroll
     for (int j = 0; j < kMaxExtent; j++)
       if (j < N) a[j] = ld<double>(F + base + j * S);
-    for (int k = 0; k < N; k++) {
-      cx<double> acc = mk<double>(0.0, 0.0);
+    for (int k = 0; k < N; k += 2) {
+      const int k2 = k + 1 < N ? k + 1 : k;
+      cx<double> acc = mk<double>(0.0, 0.0), acc2 = acc;
+      int jk = 0, jk2 = 0;  // j k mod N
 #pragma unroll
       for (int j = 0; j < kMaxExtent; j++) {
         if (j < N) {
-          const double2 w = tw[3][(j * k) % N];
+          const double2 w = tw[3][jk], w2 = tw[3][jk2];
           acc = cadd(acc, cmul(a[j], mk<double>(w.x, -w.y)));
+          acc2 = cadd(acc2, cmul(a[j], mk<double>(w2.x, -w2.y)));
+          jk += k;
+          if (jk >= N) jk -= N;
+          jk2 += k2;
+          if (jk2 >= N) jk2 -= N;
         }
       }
       st<double>(F + base + k * S, acc);
+      if (k2 != k) st<double>(F + base + k2 * S, acc2);
     }
   }
   __syncthreads();
@@ -501,6 +546,12 @@ static int batch_geometry(const size_t dims[4], size_t nlat, size_t max_iters, B
     V *= (long long)dims[mu];
   }
   g->V = V;
+  // division multipliers (extents are at least 2, strides past the first too)
+  auto mul = [](long long d) { return (unsigned)(0xFFFFFFFFull / (unsigned long long)d) + 1u; };
+  for (int mu = 0; mu < 4; mu++) {
+    g->mL[mu] = mul(g->L[mu]);
+    g->mS[mu] = mu ? mul(g->S[mu]) : 0u;
+  }
   return 0;
 }
 
