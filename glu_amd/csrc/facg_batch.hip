@@ -162,68 +162,96 @@ template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict_
   __syncthreads();
 }
 
-// The x, y and z transforms of one time slice of one field (a slab of
+// One direct DFT of a pencil held in registers: out_k = sum_j a_j w_jk,
+// j ascending, w_jk = tw[j k mod N] forward (sign -1) and its conjugate
+// backward; outputs k and k + 1 are summed together, two independent
+// chains over the same inputs.  ld(j) reads input j, st(k, v) writes
+// output k; every input is read before the first output is written, so a
+// pencil may be transformed in place.
+template <typename Ld, typename St>
+__device__ __forceinline__ void pencil_dft(int N, int sign, const double2 *tw, Ld ld, St st) {
+  cx<double> a[kMaxExtent];
+#pragma unroll
+  for (int j = 0; j < kMaxExtent; j++)
+    if (j < N) a[j] = ld(j);
+  for (int k = 0; k < N; k += 2) {
+    const int k2 = k + 1 < N ? k + 1 : k;
+    cx<double> acc = mk<double>(0.0, 0.0), acc2 = acc;
+    int jk = 0, jk2 = 0;  // j k mod N
+#pragma unroll
+    for (int j = 0; j < kMaxExtent; j++) {
+      if (j < N) {
+        const double2 w = tw[jk], w2 = tw[jk2];
+        acc = cadd(acc, cmul(a[j], mk<double>(w.x, sign < 0 ? w.y : -w.y)));
+        acc2 = cadd(acc2, cmul(a[j], mk<double>(w2.x, sign < 0 ? w2.y : -w2.y)));
+        jk += k;
+        if (jk >= N) jk -= N;
+        jk2 += k2;
+        if (jk2 >= N) jk2 -= N;
+      }
+    }
+    st(k, acc);
+    if (k2 != k) st(k2, acc2);
+  }
+}
+
+// The x, y and z transforms of the time slices of the fields (slabs of
 // Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
 // the stage holds: the slabs cross HBM once for the three axes instead of
 // three times.  Each axis is the same direct sum as bdft_axis, in the same
-// order, from one LDS buffer into the other.  The stage holds 4096 complex
-// numbers per buffer (two of 64 KiB) with 512 threads, whose workgroup
-// holds a CU alone, and 2048 with fewer, whose workgroups share a CU.
-template <int WG> constexpr int stage_cx() { return WG >= 512 ? 4096 : 2048; }
+// order, on whole pencils in registers, in place.  Every x pencil is
+// padded by one number in LDS (site (x, y, r) at x + (Lx + 1)(y + Ly r)),
+// so that a wave's threads, one pencil each, read and write at a stride
+// that spreads over the LDS banks.  The stage holds 8192 complex numbers
+// (128 KiB) with 512 threads, whose workgroup holds a CU alone, and 4096
+// with fewer, whose workgroups share a CU.
+template <int WG> constexpr int stage_cx() { return WG >= 512 ? 8192 : 4096; }
 
-template <int WG>
-__device__ __forceinline__ void lds_dft_axis(const double2 *__restrict__ in, double2 *__restrict__ out, int npts,
-                                             int N, unsigned mN, int S, unsigned mS, int sign, const double2 *tw) {
-  // two points per thread, p and p + WG, whose sums are independent
-  // chains the SIMD interleaves (the last may repeat p, unstored)
-  for (int p = threadIdx.x; p < npts; p += 2 * WG) {
-    const int q = p + WG < npts ? p + WG : p;
-    // a point's index along the axis: (p / S) mod N, a slab's size being
-    // a multiple of S N
-    const unsigned ps = S == 1 ? (unsigned)p : bquot((unsigned)p, mS);
-    const unsigned qs = S == 1 ? (unsigned)q : bquot((unsigned)q, mS);
-    const int kp = (int)(ps - bquot(ps, mN) * (unsigned)N), kq = (int)(qs - bquot(qs, mN) * (unsigned)N);
-    const int bp = p - kp * S, bq = q - kq * S;  // the pencils' first points
-    cx<double> ap = mk<double>(0.0, 0.0), aq = mk<double>(0.0, 0.0);
-    int jp = 0, jq = 0;  // j k mod N
-#pragma unroll 4
-    for (int j = 0; j < N; j++) {
-      const double2 wp = tw[jp], wq = tw[jq];
-      const double2 xp = in[bp + j * S], xq = in[bq + j * S];
-      ap = cadd(ap, cmul(mk<double>(xp.x, xp.y), mk<double>(wp.x, sign < 0 ? wp.y : -wp.y)));
-      aq = cadd(aq, cmul(mk<double>(xq.x, xq.y), mk<double>(wq.x, sign < 0 ? wq.y : -wq.y)));
-      jp += kp;
-      if (jp >= N) jp -= N;
-      jq += kq;
-      if (jq >= N) jq -= N;
-    }
-    out[p] = make_double2(ap.re, ap.im);
-    if (q != p) out[q] = make_double2(aq.re, aq.im);
-  }
-  __syncthreads();
+// whether a slab, padded, fits the stage
+template <int WG> __device__ __forceinline__ bool xyz_staged(const BGeo &g) {
+  return (long long)(g.L[0] + 1) * g.L[1] * g.L[2] <= stage_cx<WG>();
 }
 
 // x, y, z transforms of the 4 T slabs of the 4 fields (requires
-// Lx Ly Lz <= stage_cx); lds holds two buffers of stage_cx
+// xyz_staged); the z sums are written straight back to F
 template <int WG>
 __device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g, int sign,
                                          const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
-  constexpr int kStage = stage_cx<WG>();
-  const int s3 = (int)g.S[3], per = kStage / s3, nslab = 4 * g.L[3];
-  double2 *b0 = lds, *b1 = lds + kStage;
+  const int Lx = g.L[0], Ly = g.L[1], Lz = g.L[2], s3 = (int)g.S[3], sxy = (int)g.S[2];
+  const int px = Lx + 1, per = stage_cx<WG>() / (px * Ly * Lz), nslab = 4 * g.L[3];
   for (int s0 = 0; s0 < nslab; s0 += per) {
     const int npts = min(per, nslab - s0) * s3;
     double2 *src = F + (long long)s0 * s3;  // slab (f, t) starts at (f T + t) s3
-    for (int p = threadIdx.x; p < npts; p += WG) b0[p] = src[p];
+    for (int p = threadIdx.x; p < npts; p += WG) lds[p + (int)bquot((unsigned)p, g.mL[0])] = src[p];
     __syncthreads();
-    lds_dft_axis<WG>(b0, b1, npts, g.L[0], g.mL[0], 1, 0u, sign, tw[0]);
-    lds_dft_axis<WG>(b1, b0, npts, g.L[1], g.mL[1], (int)g.S[1], g.mS[1], sign, tw[1]);
-    lds_dft_axis<WG>(b0, b1, npts, g.L[2], g.mL[2], (int)g.S[2], g.mS[2], sign, tw[2]);
-    for (int p = threadIdx.x; p < npts; p += WG) src[p] = b1[p];
-    // the next round's loads write b0, which no thread reads after the
-    // barrier that ended the z sums
+    // x: pencil q at q px, stride 1
+    for (int q = threadIdx.x; q < npts / Lx; q += WG) {
+      double2 *b = lds + q * px;
+      pencil_dft(Lx, sign, tw[0], [&](int j) { return ld<double>(b + j); },
+                 [&](int k, cx<double> v) { st<double>(b + k, v); });
+    }
+    __syncthreads();
+    // y: pencil q = x + Lx r (r = z + Lz s, s the slab of the round) at
+    // x + px Ly r, stride px
+    for (int q = threadIdx.x; q < npts / Ly; q += WG) {
+      const int r = (int)bquot((unsigned)q, g.mL[0]);
+      double2 *b = lds + (q - r * Lx) + px * Ly * r;
+      pencil_dft(Ly, sign, tw[1], [&](int j) { return ld<double>(b + j * px); },
+                 [&](int k, cx<double> v) { st<double>(b + k * px, v); });
+    }
+    __syncthreads();
+    // z: pencil q = x + Lx y + Lx Ly s at x + px (y + Ly Lz s), stride
+    // px Ly, into F at s s3 + x + Lx y, stride Lx Ly
+    for (int q = threadIdx.x; q < npts / Lz; q += WG) {
+      const int sl = (int)bquot((unsigned)q, g.mS[2]), xy = q - sl * sxy;
+      const int y = (int)bquot((unsigned)xy, g.mL[0]), x = xy - y * Lx;
+      const double2 *b = lds + x + px * (y + Ly * Lz * sl);
+      double2 *o = src + (long long)sl * s3 + xy;
+      pencil_dft(Lz, sign, tw[2], [&](int j) { return ld<double>(b + j * px * Ly); },
+                 [&](int k, cx<double> v) { st<double>(o + (long long)k * sxy, v); });
+    }
+    __syncthreads();  // the stage is free for the next round, F complete
   }
-  __syncthreads();
 }
 
 // along t, on pencils in registers: forward sums, x MAX / p^2, backward
@@ -303,7 +331,7 @@ __device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo
 template <int WG>
 __device__ __forceinline__ void bfourier_accelerate(double2 *__restrict__ F, const BGeo &g,
                                                     const double2 (*tw)[kMaxExtent], double2 *__restrict__ lds) {
-  if (g.S[3] <= stage_cx<WG>()) {  // three passes over the fields
+  if (xyz_staged<WG>(g)) {  // three passes over the fields
     bdft_xyz<WG>(F, g, -1, tw, lds);
     bdft_t_scale<WG>(F, g, tw);
     bdft_xyz<WG>(F, g, +1, tw, lds);
@@ -420,7 +448,7 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
                                                           int keep_failed) {
   __shared__ double sh[3][kMaxWgWaves];
   __shared__ double2 tw[4][kMaxExtent];
-  __shared__ double2 stage[2 * stage_cx<WG>()];
+  __shared__ double2 stage[stage_cx<WG>()];
   const int b = blockIdx.x;
   if (b >= nlat) return;
   const long long V = g.V;

@@ -170,12 +170,14 @@ def test_continuation_with_stale_slots_and_sentinel():
 
 
 
-@pytest.mark.parametrize("wg", ["128", "256"])
-def test_other_workgroup_sizes(wg, monkeypatch):
+@pytest.mark.parametrize("wg,dims", [("128", (16, 12, 12, 4)), ("256", (16, 12, 12, 4)),
+                                     ("256", (16, 16, 16, 2))])
+def test_other_workgroup_sizes(wg, dims, monkeypatch):
     """The 128- and 256-thread forms of the kernel (GLU_GFIX_BATCH_WG)
     against the oracle: only the summation order changes.  Their LDS stage
-    holds 2048 sites, so time slices of 2304 take the nine-pass form."""
-    dims = (16, 12, 12, 4)
+    holds 4096 numbers: time slices of 16x12x12 (17x12x12 padded) are
+    staged one at a time, those of 16^3 (17x16x16) take the nine-pass form
+    of the transforms."""
     monkeypatch.setenv("GLU_GFIX_BATCH_WG", wg)
     lats, plaq = copies(dims, 2, seed0=700)
     its, th, tr, out = run_batch(dims, lats)
