@@ -18,10 +18,11 @@
 // context path streams, and written back in GLU's layout at the end with
 // the third column completed.  The fields in, sn and in_old (4 packed
 // complex fields each, Gfix/FACG.c:57-60) live in the same scratch.
-// The Fourier acceleration is a direct DFT per axis (extents up to 16): the
-// x, y, z sums of each time slice staged in LDS, the t sums with the p^2
-// factor between them on pencils in registers, so three passes over the
-// fields.  Reductions are workgroup sums in a fixed order.
+// The Fourier acceleration transforms each axis (extents up to 16) on
+// pencils in registers, by radix-2 FFT where the extent is a power of two
+// and by direct sums otherwise: the x, y, z transforms of each time slice
+// staged in LDS, the t transforms with the p^2 factor between them, so
+// three passes over the fields.  Reductions are workgroup sums in a fixed order.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -168,8 +169,69 @@ template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict_
 // chains over the same inputs.  ld(j) reads input j, st(k, v) writes
 // output k; every input is read before the first output is written, so a
 // pencil may be transformed in place.
+// bit reversal of i over log2 N bits
+template <int N> __device__ __forceinline__ constexpr int brev(int i) {
+  int r = 0;
+  for (int n = N; n > 1; n >>= 1) {
+    r = (r << 1) | (i & 1);
+    i >>= 1;
+  }
+  return r;
+}
+
+// In place, the DFT of N = 2, 4, 8 or 16 numbers in registers by radix-2
+// decimation in time (unnormalised, the sign convention of the direct
+// sums): the inputs taken in bit-reversed order, then log2 N stages of
+// butterflies with twiddles tw[j N / len] from the length-N table, the
+// j = 0 twiddle (1) not multiplied.  N log2 N / 2 complex products instead
+// of the direct sums' N^2; every index is a compile-time constant.
+template <int N>
+__device__ __forceinline__ void fft_pow2(cx<double> (&a)[kMaxExtent], int sign, const double2 *tw) {
+  cx<double> b[N];
+#pragma unroll
+  for (int i = 0; i < N; i++) b[i] = a[brev<N>(i)];
+#pragma unroll
+  for (int len = 2; len <= N; len <<= 1) {
+#pragma unroll
+    for (int i = 0; i < N; i += len) {
+#pragma unroll
+      for (int j = 0; j < len / 2; j++) {
+        cx<double> t = b[i + j + len / 2];
+        if (j > 0) {
+          const double2 w = tw[j * (N / len)];
+          t = cmul(t, mk<double>(w.x, sign < 0 ? w.y : -w.y));
+        }
+        const cx<double> u = b[i + j];
+        b[i + j] = cadd(u, t);
+        b[i + j + len / 2] = csub(u, t);
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; i++) a[i] = b[i];
+}
+
+template <int N, typename Ld, typename St>
+__device__ __forceinline__ void pencil_fft(int sign, const double2 *tw, Ld ld, St st) {
+  cx<double> a[kMaxExtent];
+#pragma unroll
+  for (int j = 0; j < N; j++) a[j] = ld(j);
+  fft_pow2<N>(a, sign, tw);
+#pragma unroll
+  for (int k = 0; k < N; k++) st(k, a[k]);
+}
+
+// pencil_dft: power-of-two extents by fft_pow2, the others by the direct
+// sums
 template <typename Ld, typename St>
 __device__ __forceinline__ void pencil_dft(int N, int sign, const double2 *tw, Ld ld, St st) {
+  switch (N) {
+    case 2: pencil_fft<2>(sign, tw, ld, st); return;
+    case 4: pencil_fft<4>(sign, tw, ld, st); return;
+    case 8: pencil_fft<8>(sign, tw, ld, st); return;
+    case 16: pencil_fft<16>(sign, tw, ld, st); return;
+    default: break;
+  }
   cx<double> a[kMaxExtent];
 #pragma unroll
   for (int j = 0; j < kMaxExtent; j++)
@@ -198,8 +260,9 @@ __device__ __forceinline__ void pencil_dft(int N, int sign, const double2 *tw, L
 // The x, y and z transforms of the time slices of the fields (slabs of
 // Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
 // the stage holds: the slabs cross HBM once for the three axes instead of
-// three times.  Each axis is the same direct sum as bdft_axis, in the same
-// order, on whole pencils in registers, in place.  Every x pencil is
+// three times.  Each axis is transformed by pencil_dft on whole pencils
+// in registers, in place (the nine-pass form, bdft_axis, takes the direct
+// sums for every extent).  Every x pencil is
 // padded by one number in LDS (site (x, y, r) at x + (Lx + 1)(y + Ly r)),
 // so that a wave's threads, one pencil each, read and write at a stride
 // that spreads over the LDS banks.  The stage holds 8192 complex numbers
@@ -260,9 +323,43 @@ __device__ __forceinline__ void bdft_xyz(double2 *__restrict__ F, const BGeo &g,
 // the backward sums (the thread's own stores, still in cache), so that
 // every register array is indexed by constants.  Outputs k and k + 1 are
 // summed together, two independent chains over the same inputs.
+template <int N, int WG>
+__device__ __forceinline__ void bdft_t_scale_fft(double2 *__restrict__ F, const BGeo &g, const double2 (*tw)[kMaxExtent]) {
+  const long long S = g.S[3], V = g.V;
+  const double mx = 4.0 * 4 / (double)V;
+  for (unsigned p = threadIdx.x; p < 4 * S; p += WG) {
+    const unsigned f = bquot(p, g.mS[3]), q = p - f * (unsigned)S;
+    const long long base = (long long)f * V + q;
+    int x[4];
+    bcoords(g, q, x);
+    const double kxyz = tw[0][x[0]].x + tw[1][x[1]].x + tw[2][x[2]].x;
+    cx<double> a[kMaxExtent];
+#pragma unroll
+    for (int j = 0; j < N; j++) a[j] = ld<double>(F + base + j * S);
+    fft_pow2<N>(a, -1, tw[3]);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      const double kc = kxyz + tw[3][k].x;  // sum of cos(2 pi n_mu / L_mu), mu in order
+      const double fac = mx / ((kc == 4.0) ? 1.0 : 2.0 * (4.0 - kc));
+      a[k] = mk<double>(a[k].re * fac, a[k].im * fac);
+    }
+    fft_pow2<N>(a, +1, tw[3]);
+#pragma unroll
+    for (int k = 0; k < N; k++) st<double>(F + base + k * S, a[k]);
+  }
+  __syncthreads();
+}
+
 template <int WG>
 __device__ __forceinline__ void bdft_t_scale(double2 *__restrict__ F, const BGeo &g, const double2 (*tw)[kMaxExtent]) {
   const int N = g.L[3];
+  switch (N) {  // powers of two: the FFT, both directions in registers
+    case 2: bdft_t_scale_fft<2, WG>(F, g, tw); return;
+    case 4: bdft_t_scale_fft<4, WG>(F, g, tw); return;
+    case 8: bdft_t_scale_fft<8, WG>(F, g, tw); return;
+    case 16: bdft_t_scale_fft<16, WG>(F, g, tw); return;
+    default: break;
+  }
   const long long S = g.S[3], V = g.V;
   const double mx = 4.0 * 4 / (double)V;
   for (unsigned p = threadIdx.x; p < 4 * S; p += WG) {
