@@ -526,6 +526,12 @@ template <int WG> __device__ __forceinline__ void begauge(double2 *__restrict__ 
   __syncthreads();
 }
 
+__device__ __forceinline__ void swap_fields(double2 *&a, double2 *&b) {
+  double2 *t = a;
+  a = b;
+  b = t;
+}
+
 struct BTrace {
   double *rows;  // 3 per row
   long long cap, n;
@@ -596,9 +602,12 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
     double alpha = approx_minimum(als, val);
     begauge<WG>(U, in, g, alpha);
     tr.rec(theta, alpha, 0.0);
-    for (long long j = threadIdx.x; j < 4 * V; j += WG) sn[j] = old[j] = in[j];
+    for (long long j = threadIdx.x; j < 4 * V; j += WG) sn[j] = in[j];
+    // in_old <- in: the two buffers trade roles instead of copying, and
+    // the next derivative is written into the other
+    swap_fields(in, old);
     double s2[2];
-    bpr_sums<WG>(in, in, g, s2, sh);  // <in, in> (sum_DER3)
+    bpr_sums<WG>(old, old, g, s2, sh);  // <in, in> (sum_DER3)
     double inold = s2[0] + stale_insum;
     long long loc = 0;
     double insum = 0.0, probe[3] = {0.0, 0.0, 0.0};
@@ -615,8 +624,8 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
       for (long long j = threadIdx.x; j < 4 * V; j += WG) {
         const double2 a = in[j], s = sn[j];
         sn[j] = make_double2(a.x + beta * s.x, a.y + beta * s.y);
-        old[j] = a;
       }
+      swap_fields(in, old);  // in_old <- in
       __syncthreads();
       if (theta > kCgTolB) {
         bprobes<WG>(U, sn, g, probe, sh);
