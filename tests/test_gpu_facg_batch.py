@@ -92,7 +92,10 @@ def oracle_alts(W, dims, **kw):
 
 @pytest.mark.parametrize("dims,n", [((8, 8, 8, 8), 4), ((6, 6, 6, 6), 3), ((4, 6, 8, 12), 2),
                                     ((5, 7, 6, 9), 2),      # odd extents
-                                    ((16, 12, 12, 4), 2)])  # time slices of 2304 sites
+                                    ((16, 12, 12, 4), 2),   # time slices of 2304 sites
+                                    # every power-of-two transform (2, 4, 16) on
+                                    # the staged axes and on t
+                                    ((2, 16, 4, 6), 2), ((4, 2, 6, 16), 2)])
 def test_batch_trajectories_and_final_state(dims, n):
     lats, plaq = copies(dims, n)
     its, th, tr, out = run_batch(dims, lats)
