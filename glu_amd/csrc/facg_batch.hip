@@ -44,8 +44,9 @@ constexpr double kPl1 = 0.15, kPl2 = 0.30;  // Gfix/line_search.c:34-35
 constexpr long long kScratchPerSite = 36;
 constexpr int kMaxSegments = 100;  // continuation segments; GLU does not bound them
 
-// mL, mS: multipliers for division by the extents and strides without a
-// divide instruction (no GPU has one for integers): for 2 <= d and
+// mL, mS: multipliers for division by the extents and strides (CDNA has
+// no integer divide instruction; a division by a runtime value expands to
+// a few dozen instructions, 64-bit ones to many more): for 2 <= d and
 // x < 2^32 / d, x / d == umulhi(x, floor((2^32 - 1) / d) + 1).  Lattices
 // of at most 16^4 sites keep every quotient here in that range.
 struct BGeo {
@@ -130,37 +131,6 @@ template <int WG> __device__ __forceinline__ double bderiv(const double2 *__rest
   }
   wg_sum<WG, 1>(v, sh);
   return v[0];
-}
-
-// one direct DFT along axis mu of every pencil of the 4 fields (sign -1
-// forward, +1 backward, unnormalised); tw[k] = exp(-2 pi i k / N)
-template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict__ F, const BGeo &g, int mu, int sign, const double2 *tw) {
-  const int N = g.L[mu];
-  const unsigned S = (unsigned)g.S[mu], V = (unsigned)g.V;
-  const unsigned np = V / N;  // pencils per field
-  for (unsigned p = threadIdx.x; p < 4 * np; p += WG) {
-    const unsigned f = p / np, q = p % np;
-    const long long base = (long long)f * V + (q % S) + (q / S) * S * N;
-    cx<double> a[kMaxExtent];
-#pragma unroll
-    for (int j = 0; j < kMaxExtent; j++)
-      if (j < N) a[j] = ld<double>(F + base + j * S);
-    for (int k = 0; k < N; k++) {
-      cx<double> acc = mk<double>(0.0, 0.0);
-      int jk = 0;  // j k mod N
-#pragma unroll
-      for (int j = 0; j < kMaxExtent; j++) {
-        if (j < N) {
-          const double2 w = tw[jk];
-          acc = cadd(acc, cmul(a[j], mk<double>(w.x, sign < 0 ? w.y : -w.y)));
-          jk += k;
-          if (jk >= N) jk -= N;
-        }
-      }
-      st<double>(F + base + k * S, acc);
-    }
-  }
-  __syncthreads();
 }
 
 // One direct DFT of a pencil held in registers: out_k = sum_j a_j w_jk,
@@ -257,12 +227,27 @@ __device__ __forceinline__ void pencil_dft(int N, int sign, const double2 *tw, L
   }
 }
 
+// one DFT along axis mu of every pencil of the 4 fields (sign -1 forward,
+// +1 backward, unnormalised; tw[k] = exp(-2 pi i k / N)) in place in F:
+// the nine-pass form, for slabs beyond the LDS stage
+template <int WG> __device__ __forceinline__ void bdft_axis(double2 *__restrict__ F, const BGeo &g, int mu, int sign, const double2 *tw) {
+  const int N = g.L[mu];
+  const unsigned S = (unsigned)g.S[mu], V = (unsigned)g.V;
+  const unsigned np = V / N;  // pencils per field
+  for (unsigned p = threadIdx.x; p < 4 * np; p += WG) {
+    const unsigned f = p / np, q = p % np;
+    double2 *b = F + (long long)f * V + (q % S) + (q / S) * S * N;
+    pencil_dft(N, sign, tw, [&](int j) { return ld<double>(b + (long long)j * S); },
+               [&](int k, cx<double> v) { st<double>(b + (long long)k * S, v); });
+  }
+  __syncthreads();
+}
+
 // The x, y and z transforms of the time slices of the fields (slabs of
 // Lx Ly Lz sites, contiguous in F) run in LDS, as many slabs at a time as
 // the stage holds: the slabs cross HBM once for the three axes instead of
 // three times.  Each axis is transformed by pencil_dft on whole pencils
-// in registers, in place (the nine-pass form, bdft_axis, takes the direct
-// sums for every extent).  Every x pencil is
+// in registers, in place.  Every x pencil is
 // padded by one number in LDS (site (x, y, r) at x + (Lx + 1)(y + Ly r)),
 // so that a wave's threads, one pencil each, read and write at a stride
 // that spreads over the LDS banks.  The stage holds 8192 complex numbers
