@@ -133,6 +133,36 @@ template <int WG> __device__ __forceinline__ double bderiv(const double2 *__rest
   return v[0];
 }
 
+// the site's own-link terms of Delta, left by begauge_own in F (the 8
+// accumulators as 4 complex planes), read back by bderiv_rest
+__device__ __forceinline__ void own_store(double2 *__restrict__ F, long long V, long long i, const DerivAcc<double> &d) {
+  F[i] = make_double2(d.R0, d.R1);
+  F[V + i] = make_double2(d.I1, d.R2);
+  F[2 * V + i] = make_double2(d.I2, d.R3);
+  F[3 * V + i] = make_double2(d.R4, d.I4);
+}
+
+// bderiv after begauge_own: the own-link terms from F, the backward
+// neighbours' links added, packed into F in place (each site by its own
+// thread).  The terms are summed in another order than bderiv's.
+template <int WG> __device__ __forceinline__ double bderiv_rest(const double2 *__restrict__ U, double2 *__restrict__ F,
+                                                                const BGeo &g, double (*sh)[kMaxWgWaves]) {
+  double v[1] = {0.0};
+  const long long V = g.V;
+  for (long long i = threadIdx.x; i < V; i += WG) {
+    int x[4];
+    bcoords(g, i, x);
+    DerivAcc<double> d;
+    const double2 p0 = F[i], p1 = F[V + i], p2 = F[2 * V + i], p3 = F[3 * V + i];
+    d.R0 = p0.x, d.R1 = p0.y, d.I1 = p1.x, d.R2 = p1.y, d.I2 = p2.x, d.R3 = p2.y, d.R4 = p3.x, d.I4 = p3.y;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) d.add_bwd(bload(U, V, bbwd(g, i, x, mu), mu));
+    v[0] += d.store(F, V, i);
+  }
+  wg_sum<WG, 1>(v, sh);
+  return v[0];
+}
+
 // One direct DFT of a pencil held in registers: out_k = sum_j a_j w_jk,
 // j ascending, w_jk = tw[j k mod N] forward (sign -1) and its conjugate
 // backward; outputs k and k + 1 are summed together, two independent
@@ -517,6 +547,29 @@ __device__ __forceinline__ void swap_fields(double2 *&a, double2 *&b) {
   b = t;
 }
 
+// begauge, and the updated links' own-site terms of the next Delta into
+// F (own_store): the next derivative then reads only the backward
+// neighbours' links (bderiv_rest)
+template <int WG>
+__device__ __forceinline__ void begauge_own(double2 *__restrict__ U, const double2 *__restrict__ D, double2 *__restrict__ F,
+                                            const BGeo &g, double alpha) {
+  const long long V = g.V;
+  for (long long i = threadIdx.x; i < V; i += WG) {
+    int x[4];
+    bcoords(g, i, x);
+    const su3<double> A = bgauge(D, V, i, alpha);
+    DerivAcc<double> d;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+      const su3<double> m = gtransform<double>(A, bload(U, V, i, mu), bgauge(D, V, bfwd(g, i, x, mu), alpha));
+      bstore(U, V, i, mu, m);
+      d.add_own(m);
+    }
+    own_store(F, V, i, d);
+  }
+  __syncthreads();
+}
+
 struct BTrace {
   double *rows;  // 3 per row
   long long cap, n;
@@ -597,8 +650,9 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
     long long loc = 0;
     double insum = 0.0, probe[3] = {0.0, 0.0, 0.0};
     int did_ls = 0;
+    int own = 0;  // whether begauge_own left the own-link terms in `in`
     for (;;) {
-      theta = bderiv<WG>(U, in, g, sh) * norm;
+      theta = (own ? bderiv_rest<WG>(U, in, g, sh) : bderiv<WG>(U, in, g, sh)) * norm;
       bfourier_accelerate<WG>(in, g, tw, stage);
       bpr_sums<WG>(in, old, g, s2, sh);
       insum = s2[0];
@@ -621,7 +675,8 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
         alpha = gf;
         did_ls = 0;
       }
-      begauge<WG>(U, sn, g, alpha);
+      begauge_own<WG>(U, sn, in, g, alpha);  // `in` is free until the next derivative
+      own = 1;
       tr.rec(theta, alpha, beta);
       loc++;
       if (!(theta > acc && loc < max_iters)) break;

@@ -114,6 +114,28 @@ template <typename T> struct DerivAcc {
     R4 += s.e[5].re - s.e[7].re - q.e[5].re + q.e[7].re;
     I4 += s.e[5].im + s.e[7].im - q.e[5].im - q.e[7].im;
   }
+  // the two halves of add, for a caller that has the site's own links
+  // (q) and its backward neighbours' (s) at different times
+  __device__ __forceinline__ void add_own(const su3<T> &q) {
+    R0 += T(-2) * q.e[0].im + q.e[4].im + q.e[8].im;
+    R1 += q.e[3].re - q.e[1].re;
+    I1 += -q.e[1].im - q.e[3].im;
+    R2 += q.e[6].re - q.e[2].re;
+    I2 += -q.e[2].im - q.e[6].im;
+    R3 += T(-2) * q.e[4].im + q.e[0].im + q.e[8].im;
+    R4 += q.e[7].re - q.e[5].re;
+    I4 += -q.e[5].im - q.e[7].im;
+  }
+  __device__ __forceinline__ void add_bwd(const su3<T> &s) {
+    R0 += T(2) * s.e[0].im - s.e[4].im - s.e[8].im;
+    R1 += s.e[1].re - s.e[3].re;
+    I1 += s.e[1].im + s.e[3].im;
+    R2 += s.e[2].re - s.e[6].re;
+    I2 += s.e[2].im + s.e[6].im;
+    R3 += T(2) * s.e[4].im - s.e[0].im - s.e[8].im;
+    R4 += s.e[5].re - s.e[7].re;
+    I4 += s.e[5].im + s.e[7].im;
+  }
   // local tr Delta^2 (lin_derivs.c:180-183)
   __device__ __forceinline__ double tr2() const {
     const double r0 = R0, r3 = R3;
