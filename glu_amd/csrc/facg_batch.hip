@@ -769,7 +769,7 @@ int facg_batch_run(const size_t dims[4], size_t nlat, double *const *lat, double
     rc = gfix_fail(-2, "glu_gfix_facg_batch: pointer upload failed");
   } else {
     // threads per lattice: 512 unless GLU_GFIX_BATCH_WG picks 128 or 256
-    // (512 8^4 lattices: 8670, 8550 and 6590 configs/s at 512, 256 and 128)
+    // (512 8^4 lattices: 8950, 8800 and 6850 configs/s at 512, 256 and 128)
     int wg = 512;
     if (const char *e = getenv("GLU_GFIX_BATCH_WG")) wg = atoi(e);
     auto launch = [&](auto kernel, int threads) {
