@@ -17,7 +17,9 @@
 // columns of each link in structure-of-arrays planes, the layout the
 // context path streams, and written back in GLU's layout at the end with
 // the third column completed.  The fields in, sn and in_old (4 packed
-// complex fields each, Gfix/FACG.c:57-60) live in the same scratch.
+// complex fields each, Gfix/FACG.c:57-60) live in the same scratch, sn
+// twice: the CG update is formed inside the line-search probes, which
+// write the new direction beside the old one.
 // The Fourier acceleration transforms each axis (extents up to 16) on
 // pencils in registers, by radix-2 FFT where the extent is a power of two
 // and by direct sums otherwise: the x, y, z transforms of each time slice
@@ -40,8 +42,8 @@ constexpr int kMaxExtent = 16;
 constexpr double kCgTolB = 5e-11;     // CG_TOL, Headers/GLU_definitions.h:449-455
 constexpr double kPl1 = 0.15, kPl2 = 0.30;  // Gfix/line_search.c:34-35
 // scratch per site of each lattice, in complex numbers: the links'
-// columns 0 and 1 (24), the fields in, sn and in_old (4 each)
-constexpr long long kScratchPerSite = 36;
+// columns 0 and 1 (24), the fields in, in_old and two of sn (4 each)
+constexpr long long kScratchPerSite = 40;
 constexpr int kMaxSegments = 100;  // continuation segments; GLU does not bound them
 
 // mL, mS: multipliers for division by the extents and strides (CDNA has
@@ -526,6 +528,53 @@ template <int WG> __device__ __forceinline__ void bprobes(const double2 *__restr
   for (int k = 0; k < 3; k++) out[k] = v[k];
 }
 
+// bprobes along the new direction sn + beta... : D = I + beta S formed
+// where it is read (at each site and its forward neighbours, with the
+// arithmetic of the CG update), and each site's own value written to N,
+// the next direction -- the CG update folded into the probes.  S is only
+// read here, so no thread waits for another's update.
+template <int WG>
+__device__ __forceinline__ void bprobes_cg(const double2 *__restrict__ U, const double2 *__restrict__ I,
+                                           const double2 *__restrict__ S, double2 *__restrict__ N, double beta,
+                                           const BGeo &g, double out[3], double (*sh)[kMaxWgWaves]) {
+  double v[3] = {0.0, 0.0, 0.0};
+  const long long V = g.V;
+  auto dir = [&](long long k) {
+    const double2 a = I[k], s = S[k];
+    return make_double2(a.x + beta * s.x, a.y + beta * s.y);
+  };
+  for (long long i = threadIdx.x; i < V; i += WG) {
+    int x[4];
+    bcoords(g, i, x);
+    const double2 d0 = dir(i), d1 = dir(V + i), d2 = dir(2 * V + i), d3 = dir(3 * V + i);
+    N[i] = d0;
+    N[V + i] = d1;
+    N[2 * V + i] = d2;
+    N[3 * V + i] = d3;
+    const cx<double> f0 = mk<double>(d0.x, d0.y), f1 = mk<double>(d1.x, d1.y), f2 = mk<double>(d2.x, d2.y),
+                     f3 = mk<double>(d3.x, d3.y);
+    double acc1 = 0.0, acc2 = 0.0;
+#pragma unroll 1
+    for (int mu = 0; mu < 4; mu++) {
+      const su3<double> u = bload(U, V, i, mu);
+      v[0] += u.e[0].re + u.e[4].re + u.e[8].re;
+      cx<double> au1[6], au2[6];
+      ab_rows01<double>(gauge_matrix<double>(f0, f1, f2, f3, kPl1), u, au1);
+      ab_rows01<double>(gauge_matrix<double>(f0, f1, f2, f3, kPl2), u, au2);
+      const long long f = bfwd(g, i, x, mu);
+      const double2 n0 = dir(f), n1 = dir(V + f), n2 = dir(2 * V + f), n3 = dir(3 * V + f);
+      const cx<double> g0 = mk<double>(n0.x, n0.y), g1 = mk<double>(n1.x, n1.y), g2 = mk<double>(n2.x, n2.y),
+                       g3 = mk<double>(n3.x, n3.y);
+      acc1 += retrace_rows01_dag<double>(au1, gauge_matrix<double>(g0, g1, g2, g3, kPl1));
+      acc2 += retrace_rows01_dag<double>(au2, gauge_matrix<double>(g0, g1, g2, g3, kPl2));
+    }
+    v[1] += acc1;
+    v[2] += acc2;
+  }
+  wg_sum<WG, 3>(v, sh);
+  for (int k = 0; k < 3; k++) out[k] = v[k];
+}
+
 // U_mu(x) <- g(x) U_mu(x) g(x + mu)^dag with g from D at step alpha, in
 // place: g depends on D only, and each link is written by its own site
 // (Gfix/line_search.c:60-97)
@@ -584,7 +633,7 @@ struct BTrace {
 };
 
 // FACG of lattice blockIdx.x: links U (GLU layout, V sites), scratch
-// fields in, sn, old (4 V complex each).  Writes its iteration count (GLU's
+// fields in, in_old and two of sn (4 V complex each).  Writes its iteration count (GLU's
 // return value) and final theta.
 template <int WG>
 __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *const *__restrict__ lat, double2 *__restrict__ scratch,
@@ -600,7 +649,7 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
   const long long V = g.V;
   double2 *glu = lat[b];
   double2 *U = scratch + (long long)b * V * kScratchPerSite;  // the links, columns 0 and 1
-  double2 *in = U + 24 * V, *sn = in + 4 * V, *old = in + 8 * V;
+  double2 *in = U + 24 * V, *sn = in + 4 * V, *old = in + 8 * V, *sn2 = in + 12 * V;
   for (long long j = threadIdx.x; j < 4 * V; j += WG) {
     const long long i = j >> 2;
     const int mu = (int)(j & 3);
@@ -660,21 +709,23 @@ __global__ __launch_bounds__(WG, WG >= 256 ? 2 : 1) void k_facg_batch(double2 *c
       if (beta < 0.0) beta = 0.0;
       if (theta < kCgTolB) beta = insum / inold;  // Fletcher-Reeves below CG_TOL
       inold = insum;
-      for (long long j = threadIdx.x; j < 4 * V; j += WG) {
-        const double2 a = in[j], s = sn[j];
-        sn[j] = make_double2(a.x + beta * s.x, a.y + beta * s.y);
-      }
-      swap_fields(in, old);  // in_old <- in
-      __syncthreads();
       if (theta > kCgTolB) {
-        bprobes<WG>(U, sn, g, probe, sh);
+        // sn <- in + beta sn inside the probes, into the other buffer
+        bprobes_cg<WG>(U, in, sn, sn2, beta, g, probe, sh);
+        swap_fields(sn, sn2);
         for (int k = 0; k < 3; k++) val[k] = -probe[k];
         alpha = approx_minimum(als, val);
         did_ls = 1;
       } else {
+        for (long long j = threadIdx.x; j < 4 * V; j += WG) {
+          const double2 a = in[j], s = sn[j];
+          sn[j] = make_double2(a.x + beta * s.x, a.y + beta * s.y);
+        }
+        __syncthreads();
         alpha = gf;
         did_ls = 0;
       }
+      swap_fields(in, old);  // in_old <- in
       begauge_own<WG>(U, sn, in, g, alpha);  // `in` is free until the next derivative
       own = 1;
       tr.rec(theta, alpha, beta);
@@ -769,7 +820,7 @@ int facg_batch_run(const size_t dims[4], size_t nlat, double *const *lat, double
     rc = gfix_fail(-2, "glu_gfix_facg_batch: pointer upload failed");
   } else {
     // threads per lattice: 512 unless GLU_GFIX_BATCH_WG picks 128 or 256
-    // (512 8^4 lattices: 8950, 8800 and 6850 configs/s at 512, 256 and 128)
+    // (512 8^4 lattices: 9350, 8940 and 7080 configs/s at 512, 256 and 128)
     int wg = 512;
     if (const char *e = getenv("GLU_GFIX_BATCH_WG")) wg = atoi(e);
     auto launch = [&](auto kernel, int threads) {

@@ -105,8 +105,8 @@ bool same_shape(const size_t a[4], const size_t b[4]) {
 // extents the shared small-lattice launch takes (facg_batch.hip)
 constexpr size_t kBatchMaxExtent = 16;
 constexpr size_t kBatchMaxLattices = 65535;
-// its scratch per site of each lattice: 36 complex numbers
-constexpr size_t kBatchScratchPerSite = 36 * 16;
+// its scratch per site of each lattice: 40 complex numbers
+constexpr size_t kBatchScratchPerSite = 40 * 16;
 
 // a shared launch in flight: its requests and the flag its host thread
 // clears when the results are in

@@ -191,7 +191,7 @@ int glu_gfix_luxury(glu_gfix_ctx *ctx, int method, double gf_tune,
    bound them).  Landau()'s random restarts are left to the caller
    (glu_gfix_landau on the failed ones).  Waits for the device's pending
    work, runs on a stream of its own and returns when every lattice has
-   stopped; allocates 576 B per site per lattice of device scratch for the
+   stopped; allocates 640 B per site per lattice of device scratch for the
    call.  For lattices whose iterations are bound by launch latency (8^4:
    one iteration of the context path takes about 75 us). */
 int glu_gfix_facg_batch(const size_t dims[4], size_t nlat, double *links_dev, double gf_tune,
@@ -427,7 +427,7 @@ int glu_gfix_server_configure(glu_gfix_server *srv, double switch_theta,
    random transform skip the contexts.  Every queued request of one shape,
    gf_tune, accuracy and max_iters is fixed in one glu_gfix_facg_batch
    launch (one workgroup per lattice) on a host thread of the server; the
-   launch takes one slot and its scratch (576 B per site per lattice)
+   launch takes one slot and its scratch (640 B per site per lattice)
    counts against the budget, and while it runs, later requests of that
    key gather for the next.  The other requests are admitted FIFO among
    themselves past the ones gathering.  A lattice whose
